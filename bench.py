@@ -1,0 +1,156 @@
+"""Benchmark: 28-qubit random brickwork circuit (depth 20) on a complex128 state.
+
+A step applies the whole circuit (1,950 gates: Rx, Ry, Rz on every qubit per layer,
+then CNOTs on even / odd pairs) to the 4 GiB state resident in HBM. Prints one JSON
+line with whole-job gate applications per second, the dominant kernel's roofline
+(live HIP-event timing on the library's stream) and the CPU oracle timed on a
+bounded sample of the same workload on this host.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "yao-rs_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+NBITS = 28
+DEPTH = 20
+SEED = 20
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def gate_bytes(e, nbits, amp_bytes=16):
+    """Algorithmic HBM bytes of one gate applied on its own: every amplitude it can
+    change is read and written once (2^n for a 1-qubit gate, 2^(n-c) with c controls)."""
+    return 2 * amp_bytes * (1 << (nbits - len(e.control_locs)))
+
+
+def cpu_baseline(els, nbits):
+    """Oracle (single-threaded C++ restatement of the reference) on a bounded sample:
+    2 gates of each class at full size, extrapolated to the circuit's gate mix."""
+    from oracle import oracle as O
+    from yaohip.circuit import lower
+    buf = np.zeros(2 << nbits)
+    buf[0] = 1.0
+    buf[1:] = 0.0  # touch every page before timing
+    classes = {}
+    for e in els:
+        k = "cnot" if e.control_locs else ("rz" if e.gate.kind == 10 else "rot")
+        classes.setdefault(k, []).append(e)
+    per = {}
+    t_all = 0.0
+    for k, lst in classes.items():
+        sample = lst[:2]
+        ol = lower(sample)
+        t0 = time.perf_counter()
+        st = O.lib().orc_apply(nbits, nbits, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ol.arr, ol.n)
+        dt = time.perf_counter() - t0
+        assert st == 0
+        per[k] = dt / len(sample)
+        t_all += dt
+    est = sum(per[k] * len(v) for k, v in classes.items())
+    return {"value": len(els) / est, "unit": "gate-apps/s", "cores": 1, "kind": "port",
+            "sample": (f"oracle (C++ restatement of yao-rs instruct/dispatch, -O2, 1 thread) on the 28-qubit "
+                       f"state: 2 gates of each class (Rx/Ry dense, Rz diagonal, CNOT) timed "
+                       f"({t_all:.1f} s), per-class times weighted by the circuit's gate mix "
+                       f"({', '.join(f'{k}={len(v)}' for k, v in classes.items())})"),
+            "per_gate_s": per}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nbits", type=int, default=NBITS)
+    ap.add_argument("--depth", type=int, default=DEPTH)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is design only "
+                                   "(see DESIGN.md), run with --gpus 1"}))
+        sys.exit(2)
+
+    import yaohip
+    from yaohip import _abi as A
+    from yaohip.circuit import Circuit
+    from fixture_circuits import brickwork
+
+    n, depth = args.nbits, args.depth
+    els = brickwork(n, depth, SEED)
+    circ = Circuit.qubits(n, els)
+    ngates = len(els)
+    step_bytes = sum(gate_bytes(e, n) for e in els)
+    log(f"workload brickwork n={n} depth={depth} gates={ngates} algorithmic bytes/step={step_bytes/1e9:.1f} GB")
+
+    A.init()
+    name = ctypes.create_string_buffer(256)
+    A.lib().yh_device_name(name, 256)
+    log(f"device {name.value.decode()}")
+    reg = yaohip.ArrayReg.zero_state(n)
+    L = A.lib()
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        yaohip.apply_inplace(circ, reg)
+        log(f"warmup {w} {time.perf_counter() - t0:.3f} s")
+
+    A.check(L.yh_reg_timing_enable(reg._h, 1))
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        yaohip.apply_inplace(circ, reg)
+        log(f"step {s} done")
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    value = ngates * args.steps / dt
+    kstats = {}
+    for k, kn in enumerate(A.KCLASS_NAMES):
+        c, ms = ctypes.c_uint64(), ctypes.c_double()
+        A.check(L.yh_reg_timing_read(reg._h, k, ctypes.byref(c), ctypes.byref(ms)))
+        if c.value:
+            kstats[kn] = {"launches": c.value, "avg_us": 1e3 * ms.value / c.value, "total_ms": ms.value}
+    A.check(L.yh_reg_timing_enable(reg._h, 0))
+    norm = reg.norm()
+    log(f"norm after run {norm:.15f}; kernels {kstats}")
+
+    # dominant kernel: the dense 2x2 pair stream (Rx, Ry); 2 x 16 B per amplitude
+    dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
+    bytes_per_launch = 2 * 16 * (1 << n)
+    achieved = bytes_per_launch / (kstats[dom]["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": dom,
+                "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(kstats[dom]["avg_us"], 2)}
+    out = {
+        "metric": "gate-apps/s at 28q c128 (random brickwork depth 20, gate-by-gate)",
+        "value": round(value, 2), "unit": "gate-apps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"random brickwork {n}q depth {depth} complex128, |0...0>, gate-by-gate "
+                               f"(one launch per gate)", "nbits": n, "depth": depth, "gates": ngates},
+        "hbm_gbps_effective": round(step_bytes * args.steps / dt / 1e9, 1),
+        "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        del reg
+        out["cpu_baseline"] = cpu_baseline(els, n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,238 @@
+/*
+ * yaohip.h — C ABI of the MI355X (gfx950) state-vector / density-matrix backend.
+ *
+ * The entry points replace the dense simulation path of yao-rs. Every function
+ * names the reference item it stands in for (paths are relative to the yao-rs
+ * repository). Conventions carried over unchanged:
+ *   - qubit loc 0 is the MOST significant bit of a basis index (bit = n-1-loc),
+ *     src/instruct_qubit.rs:13-17;
+ *   - a multi-target gate matrix maps gate-index bit k to targets[k]
+ *     (src/instruct_qubit.rs:346-355);
+ *   - a density matrix is row-major rho[row*2^n + col] (src/density_matrix.rs:9-12);
+ *   - expectation values on a density matrix read Pauli sites as LSB-first bit
+ *     positions (src/expect.rs:55-63); every other call is MSB-first.
+ * States cross the boundary as interleaved (re, im) pairs of doubles.
+ *
+ * Every call returns a status (YH_OK = 0); yh_last_error() returns the message
+ * of the last failure on the calling thread. Handles are not thread safe: like
+ * the reference's `&mut` receivers, one handle is used by one thread at a time.
+ * Each handle owns one HIP stream; calls are synchronous with respect to the host
+ * unless stated otherwise.
+ */
+#ifndef YAOHIP_H
+#define YAOHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes: every reference panic / CircuitError maps to one ---- */
+enum yh_status {
+  YH_OK = 0,
+  YH_ERR_INVALID_ARG = 1,          /* null pointer, bad enum, bad size           */
+  YH_ERR_CONTROL_CONFIG_LEN = 2,   /* CircuitError::ControlConfigLengthMismatch  */
+  YH_ERR_LOC_OUT_OF_RANGE = 3,     /* CircuitError::LocOutOfRange                */
+  YH_ERR_OVERLAPPING_LOCS = 4,     /* CircuitError::OverlappingLocs              */
+  YH_ERR_MATRIX_SIZE = 5,          /* CircuitError::MatrixSizeMismatch           */
+  YH_ERR_NBITS_MISMATCH = 6,       /* apply.rs:128-132 register/circuit mismatch */
+  YH_ERR_MEASURE_LOC = 7,          /* measure.rs:124-133 out of range/duplicate  */
+  YH_ERR_CHANNEL_PARAM = 8,        /* noise.rs:90-176 parameter validation       */
+  YH_ERR_HIP = 9,                  /* HIP runtime failure                         */
+  YH_ERR_OUT_OF_MEMORY = 10,
+  YH_ERR_UNSUPPORTED = 11,         /* valid in the reference, not on this backend */
+  YH_ERR_DTYPE = 12,               /* operation not defined for the handle dtype  */
+  YH_ERR_RESET_VALUE = 13,         /* measure.rs:256-261 reset value too large    */
+  YH_ERR_NO_DEVICE = 14
+};
+
+enum yh_dtype { YH_C128 = 0, YH_C64 = 1 };
+
+/* Element kinds of a circuit (src/circuit.rs:33-37 CircuitElement). */
+enum yh_elem { YH_ELEM_GATE = 0, YH_ELEM_CHANNEL = 1, YH_ELEM_ANNOTATION = 2 };
+
+/* Gate kinds (src/gate.rs:7-35 enum Gate). */
+enum yh_gate_kind {
+  YH_GATE_X = 0, YH_GATE_Y, YH_GATE_Z, YH_GATE_H, YH_GATE_S, YH_GATE_T, YH_GATE_SWAP,
+  YH_GATE_PHASE, YH_GATE_RX, YH_GATE_RY, YH_GATE_RZ, YH_GATE_SQRTX, YH_GATE_SQRTY,
+  YH_GATE_SQRTW, YH_GATE_ISWAP, YH_GATE_FSIM, YH_GATE_CUSTOM
+};
+
+/* Channel kinds (src/noise.rs:16-60 enum NoiseChannel). params[] per kind:
+ *  BITFLIP{p} PHASEFLIP{p} DEPOLARIZING{p, n (0 -> nt)} PAULI{px,py,pz}
+ *  RESET{p0,p1} AMPLITUDE_DAMPING{gamma, excited_population}
+ *  PHASE_DAMPING{gamma} PHASE_AMPLITUDE_DAMPING{amplitude, phase, excited_population}
+ *  THERMAL_RELAXATION{t1, t2, time, excited_population}
+ *  COHERENT{matrix} CUSTOM{nmat Kraus matrices back to back in `matrix`}        */
+enum yh_channel_kind {
+  YH_CH_BITFLIP = 0, YH_CH_PHASEFLIP, YH_CH_DEPOLARIZING, YH_CH_PAULI, YH_CH_RESET,
+  YH_CH_AMPLITUDE_DAMPING, YH_CH_PHASE_DAMPING, YH_CH_PHASE_AMPLITUDE_DAMPING,
+  YH_CH_THERMAL_RELAXATION, YH_CH_COHERENT, YH_CH_CUSTOM
+};
+
+/* Single-site operators (src/operator.rs:8-17 enum Op). */
+enum yh_pauli { YH_OP_I = 0, YH_OP_X, YH_OP_Y, YH_OP_Z, YH_OP_P0, YH_OP_P1, YH_OP_PU, YH_OP_PD };
+
+#define YH_MAX_TARGETS 12
+#define YH_MAX_CTRLS 16
+
+/* One circuit element: PositionedGate (src/circuit.rs:103-108), PositionedChannel
+ * (:25-29) or PositionedAnnotation (:19-22). Plain data; `matrix` is only read
+ * during the call that receives it. */
+typedef struct yh_op {
+  int32_t elem;                  /* enum yh_elem                                   */
+  int32_t kind;                  /* enum yh_gate_kind or enum yh_channel_kind      */
+  int32_t nt;                    /* number of target locs (gate) / locs (channel)  */
+  int32_t nc;                    /* number of control locs                          */
+  int32_t targets[YH_MAX_TARGETS];
+  int32_t ctrls[YH_MAX_CTRLS];
+  int32_t ctrl_vals[YH_MAX_CTRLS]; /* control_configs as 0/1                        */
+  double params[4];
+  const double *matrix;          /* CUSTOM gate: dim x dim row-major interleaved;
+                                    COHERENT: one matrix; CUSTOM channel: nmat of them */
+  int32_t nmat;                  /* CUSTOM channel: number of Kraus operators      */
+  int32_t dim;                   /* matrix rows (CUSTOM gate / channel matrices)   */
+  int32_t is_diag;               /* CUSTOM gate is_diagonal flag (gate.rs:31)       */
+  int32_t nconf;                 /* control_configs length; must equal nc, else
+                                    CircuitError::ControlConfigLengthMismatch     */
+} yh_op;
+
+typedef struct yh_reg yh_reg;    /* ArrayReg on the device   (register.rs:15-18)     */
+typedef struct yh_dm yh_dm;      /* DensityMatrix on device  (density_matrix.rs:9-12) */
+
+/* ---- library ---- */
+const char *yh_last_error(void);
+/* Selects device 0 and checks that it is a gfx950 agent. */
+int yh_init(void);
+int yh_device_name(char *buf, size_t len);
+
+/* ---- ArrayReg (src/register.rs) ---- */
+int yh_reg_create(int nbits, int dtype, yh_reg **out);            /* zero_state :21-26  */
+int yh_reg_destroy(yh_reg *reg);
+int yh_reg_nbits(const yh_reg *reg, int *nbits);                  /* nqubits :87-89     */
+int yh_reg_dtype(const yh_reg *reg, int *dtype);
+int yh_reg_upload(yh_reg *reg, const double *amps, size_t n_amps); /* from_vec :65-69   */
+int yh_reg_download(const yh_reg *reg, double *amps, size_t n_amps); /* state_vec :91  */
+int yh_reg_clone(const yh_reg *src, yh_reg **out);                /* Clone / apply :150 */
+int yh_reg_copy(yh_reg *dst, const yh_reg *src);
+int yh_reg_norm(const yh_reg *reg, double *out);                  /* norm :98-104       */
+int yh_reg_normalize(yh_reg *reg);                                /* normalize :106-113 */
+int yh_reg_fidelity(const yh_reg *a, const yh_reg *b, double *out); /* fidelity :115-124 */
+/* Device address of the amplitudes (interleaved re,im; double or float by dtype). */
+int yh_reg_device_ptr(const yh_reg *reg, void **ptr);
+
+/* ---- circuit application (src/apply.rs) ---- */
+/* Validates the element list as Circuit::new does (circuit.rs:150-272) for a
+ * qubit circuit of `nbits` sites; returns the matching CircuitError status. */
+int yh_validate(int nbits, const yh_op *ops, size_t nops);
+/* apply_inplace (apply.rs:123-147): gates in order, channels and annotations
+ * skipped for a pure register. One kernel launch per gate (dispatch_arrayreg_gate
+ * :7-120). */
+int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
+/* Same semantics as yh_apply; gates are packed into passes that each stream the
+ * state through HBM once (gate runs on up to `max_fuse_qubits` qubits fused into
+ * one 2^k x 2^k block, applied with MFMA). max_fuse_qubits in [1,5]; 0 = default. */
+int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
+                   int max_fuse_qubits);
+/* dispatch_arrayreg_gate for one gate element (apply.rs:7). */
+int yh_instruct(yh_reg *reg, const yh_op *op);
+/* Host-side gate matrix, as Gate::matrix (gate.rs:63-68, 284-393); out holds
+ * dim*dim interleaved complex, dim = 2 or 4. */
+int yh_gate_matrix(const yh_op *op, double *out, int *dim);
+
+/* ---- observables (src/expect.rs, src/operator.rs) ---- */
+/* Operator polynomial, flattened: term t has coefficient (coeffs[2t], coeffs[2t+1])
+ * and term_len[t] (site, op) pairs taken in order from sites[]/ops[].
+ * OperatorString::new rules (operator.rs:27-33) are applied: sorted by site, I dropped. */
+int yh_expect(const yh_reg *reg, size_t nterms, const double *coeffs,
+              const int32_t *term_len, const int32_t *sites, const int32_t *ops,
+              double out[2]);                                       /* expect_arrayreg :7 */
+
+/* ---- measurement (src/measure.rs) ---- */
+/* probs (measure.rs:82-104): locs == NULL -> full distribution (2^n values),
+ * else marginal over locs (2^nlocs values, output bit nlocs-1-i <- locs[i]). */
+int yh_probs(const yh_reg *reg, const int32_t *locs, int nlocs, double *out);
+/* Shot sampling as sample_from_probs (:140-150) with caller-supplied uniforms:
+ * outcome[s] = first i with uniforms[s] < cumsum_i (sequential f64 cumsum), else
+ * the last index. The distribution is computed once for all shots. */
+int yh_sample(const yh_reg *reg, const int32_t *locs, int nlocs, const double *uniforms,
+              size_t nshots, uint64_t *outcomes);
+/* measure_with_postprocess (:243-274) with one uniform. post: 0 NoPostProcess,
+ * 1 ResetTo(reset_val), 2 RemoveMeasured (the register shrinks to n-nlocs qubits).
+ * outcome_bits receives nlocs bits, MSB-first over locs. */
+int yh_measure(yh_reg *reg, const int32_t *locs, int nlocs, int post, uint64_t reset_val,
+               double uniform, int32_t *outcome_bits);
+
+/* ---- DensityMatrix (src/density_matrix.rs) ---- */
+int yh_dm_create(int nbits, yh_dm **out);                          /* zero_state :19-21 */
+int yh_dm_from_reg(const yh_reg *reg, yh_dm **out);                /* from_reg :23-35   */
+int yh_dm_destroy(yh_dm *dm);
+int yh_dm_nbits(const yh_dm *dm, int *nbits);
+int yh_dm_upload(yh_dm *dm, const double *rho, size_t n_entries);
+int yh_dm_download(const yh_dm *dm, double *rho, size_t n_entries);
+/* Register::apply for DensityMatrix (:244-259): gate -> U rho U^dagger (:161-178),
+ * channel -> sum_i K_i rho K_i^dagger (:181-224). One pass over rho per element. */
+int yh_dm_apply(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
+/* Same result; each gate is fused with the 1-qubit channels that follow it on its
+ * target qubits into one superoperator pass. */
+int yh_dm_apply_fused(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
+int yh_dm_trace(const yh_dm *dm, double out[2]);                   /* trace :63-67      */
+int yh_dm_purity(const yh_dm *dm, double *out);                    /* purity :69-78     */
+/* expect_dm (expect.rs:38-72): Pauli sites are LSB-first bit positions. */
+int yh_dm_expect(const yh_dm *dm, size_t nterms, const double *coeffs,
+                 const int32_t *term_len, const int32_t *sites, const int32_t *ops,
+                 double out[2]);
+/* probs for a DensityMatrix (measure.rs:50-61, 106-122). */
+int yh_dm_probs(const yh_dm *dm, const int32_t *locs, int nlocs, double *out);
+
+/* ---- noise channels (src/noise.rs) ---- */
+/* kraus_operators (noise.rs:183-352): writes up to max_ops matrices of dim x dim
+ * interleaved complex into out, the count into *nops and the dimension into *dim. */
+int yh_channel_kraus(const yh_op *channel, double *out, int max_ops, int *nops, int *dim);
+
+/* ---- measurement hooks (no reference counterpart) ----
+ * Per-kernel-class launch timing with HIP events recorded on the handle's own
+ * stream around every launch while enabled. Classes: */
+enum yh_kclass {
+  YH_K_1Q = 0,      /* dense 2x2 pair stream            */
+  YH_K_DENSE = 1,   /* dense 2^k group, controls         */
+  YH_K_DIAG = 2,    /* diagonal phase multiply           */
+  YH_K_X = 3,       /* X pair swap                       */
+  YH_K_SWAP = 4,    /* SWAP                               */
+  YH_K_OOP = 5,     /* out-of-place dense (> 5 targets)  */
+  YH_K_FUSED = 6,   /* fused 2^k block (MFMA)            */
+  YH_K_TILE = 7,    /* multi-gate LDS tile pass          */
+  YH_K_DM = 8,      /* density-matrix block pass         */
+  YH_K_NCLASS = 9
+};
+int yh_reg_timing_enable(yh_reg *reg, int on);
+int yh_reg_timing_read(const yh_reg *reg, int kclass, uint64_t *launches, double *total_ms);
+int yh_dm_timing_enable(yh_dm *dm, int on);
+int yh_dm_timing_read(const yh_dm *dm, int kclass, uint64_t *launches, double *total_ms);
+
+/* ---- batched VQE (no reference counterpart: many ansatz instances per launch) ---- */
+typedef struct yh_vqe_batch yh_vqe_batch;
+/* Hardware-efficient ansatz: `depth` layers of Ry then Rz on every qubit, then a CZ
+ * chain CZ(i, i+1), i < n-1 (CZ = control([i],[i+1],Z), easybuild.rs:516), from
+ * |0...0>. Instance b has nq[b] qubits and depth[b] layers; its 2*nq*depth angles
+ * are angles[offset[b] ...] ordered layer, qubit, (Ry, Rz).
+ * Energy: TFIM H = -J sum Z_i Z_{i+1} - h sum X_i, open boundary.
+ * dtype YH_C64 keeps amplitudes in complex64 and reduces in FP64; YH_C128 is the
+ * all-FP64 path. */
+int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_batch **out);
+int yh_vqe_batch_destroy(yh_vqe_batch *b);
+int yh_vqe_batch_energies(yh_vqe_batch *b, int ninst, const int32_t *nq,
+                          const int32_t *depth, const int64_t *offset, const double *angles,
+                          double J, double h, double *energies);
+/* Same circuits, evaluated one instance per launch sequence (the plain per-circuit
+ * GPU path through yh_apply + yh_expect), for comparison. */
+int yh_vqe_plain_energies(int ninst, const int32_t *nq, const int32_t *depth,
+                          const int64_t *offset, const double *angles, double J, double h,
+                          int dtype, double *energies);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* YAOHIP_H */

@@ -1,0 +1,249 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker of the HIP backend. It shares the
+`struct yh_op` layout of include/yaohip.h (a data format) but no code with the
+product library.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()  # make is a no-op when the library is current
+        L = ctypes.CDLL(LIB)
+        P = ctypes.POINTER
+        d, i32, u64, vp = ctypes.c_double, ctypes.c_int32, ctypes.c_uint64, ctypes.c_void_p
+        sig = {
+            "orc_itercontrol": (ctypes.c_int, [ctypes.c_int, P(i32), P(i32), ctypes.c_int, P(u64)]),
+            "orc_gate_matrix": (ctypes.c_int, [vp, P(d), P(ctypes.c_int)]),
+            "orc_validate": (ctypes.c_int, [ctypes.c_int, vp, ctypes.c_size_t]),
+            "orc_instruct": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d),
+                                            P(i32), P(i32), ctypes.c_int]),
+            "orc_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), vp, ctypes.c_size_t]),
+            "orc_dispatch": (ctypes.c_int, [ctypes.c_int, P(d), vp]),
+            "orc_expect": (ctypes.c_int, [ctypes.c_int, P(d), ctypes.c_size_t, P(d), P(i32), P(i32), P(i32), P(d)]),
+            "orc_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
+            "orc_sample_from_probs": (u64, [P(d), ctypes.c_size_t, d]),
+            "orc_sample": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d), ctypes.c_size_t, P(u64)]),
+            "orc_measure": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, ctypes.c_int, u64, d, P(i32),
+                                           P(d)]),
+            "orc_norm": (d, [ctypes.c_int, P(d)]),
+            "orc_fidelity": (d, [ctypes.c_int, P(d), P(d)]),
+            "orc_kraus": (ctypes.c_int, [vp, P(d), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int)]),
+            "orc_superop": (ctypes.c_int, [vp, P(d), P(ctypes.c_int)]),
+            "orc_dm_from_reg": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
+            "orc_dm_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), vp, ctypes.c_size_t]),
+            "orc_dm_trace": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
+            "orc_dm_purity": (d, [ctypes.c_int, P(d)]),
+            "orc_dm_expect": (ctypes.c_int, [ctypes.c_int, P(d), ctypes.c_size_t, P(d), P(i32), P(i32), P(i32),
+                                             P(d)]),
+            "orc_dm_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
+            "orc_dm_partial_tr": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d),
+                                                 P(ctypes.c_int)]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _i(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _u(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+def _il(z):
+    return np.ascontiguousarray(np.asarray(z, dtype=np.complex128)).view(np.float64).reshape(-1).copy()
+
+
+def _cx(d):
+    return np.ascontiguousarray(d).view(np.complex128).copy()
+
+
+class OracleError(RuntimeError):
+    def __init__(self, status):
+        super().__init__(f"oracle status {status}")
+        self.status = status
+
+
+def _ck(st):
+    if st != 0:
+        raise OracleError(st)
+
+
+def itercontrol(nbits, positions, configs):
+    m = len(positions)
+    out = np.zeros(1 << (nbits - m), dtype=np.uint64)
+    _ck(lib().orc_itercontrol(nbits, _i(np.array(positions or [0], dtype=np.int32)),
+                              _i(np.array(configs or [0], dtype=np.int32)), m, _u(out)))
+    return [int(x) for x in out]
+
+
+def gate_matrix(op):
+    out = np.zeros(2 * 16, dtype=np.float64)
+    dim = ctypes.c_int()
+    _ck(lib().orc_gate_matrix(ctypes.byref(op), _d(out), ctypes.byref(dim)))
+    d = dim.value
+    return _cx(out[:2 * d * d]).reshape(d, d)
+
+
+INSTRUCT = {"1q": 0, "1q_diag": 1, "2q": 2, "2q_diag": 3, "nq": 4, "1q_controlled": 5,
+            "2q_controlled": 6, "1q_diag_controlled": 7, "2q_diag_controlled": 8, "x": 9,
+            "x_controlled": 10, "swap": 11}
+
+
+def instruct(fn, nbits, state, locs, gate=None, ctrl_locs=(), ctrl_bits=()):
+    """Call one instruct_* function of the restatement directly (in place)."""
+    s = _il(state)
+    g = _il(np.asarray(gate).reshape(-1)) if gate is not None else np.zeros(2)
+    cl = np.array(list(ctrl_locs) or [0], dtype=np.int32)
+    cb = np.array(list(ctrl_bits) or [0], dtype=np.int32)
+    _ck(lib().orc_instruct(INSTRUCT[fn], nbits, _d(s), _i(np.array(locs, dtype=np.int32)), len(locs), _d(g),
+                           _i(cl), _i(cb), len(ctrl_locs)))
+    return _cx(s)
+
+
+def apply(nbits, state, oplist, circuit_nbits=None):
+    s = _il(state)
+    _ck(lib().orc_apply(nbits, nbits if circuit_nbits is None else circuit_nbits, _d(s), oplist.arr, oplist.n))
+    return _cx(s)
+
+
+def validate(nbits, oplist):
+    return lib().orc_validate(nbits, oplist.arr, oplist.n)
+
+
+def expect(nbits, state, poly):
+    T, coeffs, tl, sites, ops = poly.flat()
+    out = np.zeros(2)
+    _ck(lib().orc_expect(nbits, _d(_il(state)), T, _d(coeffs), _i(tl), _i(sites), _i(ops), _d(out)))
+    return complex(out[0], out[1])
+
+
+def probs(nbits, state, locs=None):
+    n = (1 << nbits) if locs is None else (1 << len(locs))
+    out = np.zeros(n)
+    la = None if locs is None else np.array(list(locs) or [0], dtype=np.int32)
+    _ck(lib().orc_probs(nbits, _d(_il(state)), None if la is None else _i(la), 0 if locs is None else len(locs),
+                        _d(out)))
+    return out
+
+
+def sample_from_probs(p, r):
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    return int(lib().orc_sample_from_probs(_d(p), len(p), float(r)))
+
+
+def sample(nbits, state, locs, uniforms):
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    out = np.zeros(len(u), dtype=np.uint64)
+    la = None if locs is None else np.array(list(locs) or [0], dtype=np.int32)
+    _ck(lib().orc_sample(nbits, _d(_il(state)), None if la is None else _i(la), 0 if locs is None else len(locs),
+                         _d(u), len(u), _u(out)))
+    return out
+
+
+def measure(nbits, state, locs, post, reset_val, r):
+    """measure_with_postprocess; returns (outcome bits, new state)."""
+    s = _il(state)
+    bits = np.zeros(len(locs), dtype=np.int32)
+    newn = nbits - len(locs) if post == 2 else nbits
+    ns = np.zeros(2 << newn)
+    _ck(lib().orc_measure(nbits, _d(s), _i(np.array(locs, dtype=np.int32)), len(locs), post, reset_val, r, _i(bits),
+                          _d(ns)))
+    return [int(b) for b in bits], (_cx(ns) if post == 2 else _cx(s))
+
+
+def norm(nbits, state):
+    return lib().orc_norm(nbits, _d(_il(state)))
+
+
+def fidelity(nbits, a, b):
+    return lib().orc_fidelity(nbits, _d(_il(a)), _d(_il(b)))
+
+
+def kraus(oplist):
+    out = np.zeros(2 * 256 * 16)
+    nops, dim = ctypes.c_int(), ctypes.c_int()
+    _ck(lib().orc_kraus(ctypes.byref(oplist.arr[0]), _d(out), 256, ctypes.byref(nops), ctypes.byref(dim)))
+    d, k = dim.value, nops.value
+    return [_cx(out[2 * d * d * i: 2 * d * d * (i + 1)]).reshape(d, d) for i in range(k)]
+
+
+def superop(oplist):
+    out = np.zeros(2 * 256 * 256)
+    d2 = ctypes.c_int()
+    _ck(lib().orc_superop(ctypes.byref(oplist.arr[0]), _d(out), ctypes.byref(d2)))
+    d = d2.value
+    return _cx(out[:2 * d * d]).reshape(d, d)
+
+
+def dm_from_reg(nbits, state):
+    rho = np.zeros(2 << (2 * nbits))
+    _ck(lib().orc_dm_from_reg(nbits, _d(_il(state)), _d(rho)))
+    return _cx(rho)
+
+
+def dm_apply(nbits, rho, oplist):
+    r = _il(rho)
+    _ck(lib().orc_dm_apply(nbits, nbits, _d(r), oplist.arr, oplist.n))
+    return _cx(r)
+
+
+def dm_trace(nbits, rho):
+    out = np.zeros(2)
+    _ck(lib().orc_dm_trace(nbits, _d(_il(rho)), _d(out)))
+    return complex(out[0], out[1])
+
+
+def dm_purity(nbits, rho):
+    return lib().orc_dm_purity(nbits, _d(_il(rho)))
+
+
+def dm_expect(nbits, rho, poly):
+    T, coeffs, tl, sites, ops = poly.flat()
+    out = np.zeros(2)
+    _ck(lib().orc_dm_expect(nbits, _d(_il(rho)), T, _d(coeffs), _i(tl), _i(sites), _i(ops), _d(out)))
+    return complex(out[0], out[1])
+
+
+def dm_probs(nbits, rho, locs=None):
+    n = (1 << nbits) if locs is None else (1 << len(locs))
+    out = np.zeros(n)
+    la = None if locs is None else np.array(list(locs) or [0], dtype=np.int32)
+    _ck(lib().orc_dm_probs(nbits, _d(_il(rho)), None if la is None else _i(la), 0 if locs is None else len(locs),
+                           _d(out)))
+    return out
+
+
+def dm_partial_tr(nbits, rho, traced):
+    out = np.zeros(2 << (2 * nbits))
+    kept = ctypes.c_int()
+    _ck(lib().orc_dm_partial_tr(nbits, _d(_il(rho)), _i(np.array(list(traced) or [0], dtype=np.int32)), len(traced),
+                                _d(out), ctypes.byref(kept)))
+    k = kept.value
+    return _cx(out[:2 << (2 * k)]).reshape(1 << k, 1 << k)

@@ -1,0 +1,131 @@
+"""Builders shared by the tests: circuits from the reference's JSON fixtures
+(mirroring tests/common/mod.rs:154-240 of yao-rs) and the benchmark workloads."""
+import math
+
+import numpy as np
+
+from yaohip.circuit import (Gate, NoiseChannel, PositionedGate, channel, control, put, lower)
+
+NAMED = {
+    "X": lambda p: Gate.X, "Y": lambda p: Gate.Y, "Z": lambda p: Gate.Z, "H": lambda p: Gate.H,
+    "S": lambda p: Gate.S, "T": lambda p: Gate.T, "SWAP": lambda p: Gate.SWAP,
+    "SqrtX": lambda p: Gate.SqrtX, "SqrtY": lambda p: Gate.SqrtY, "SqrtW": lambda p: Gate.SqrtW,
+    "ISWAP": lambda p: Gate.ISWAP, "Phase": lambda p: Gate.Phase(p[0]), "Rx": lambda p: Gate.Rx(p[0]),
+    "Ry": lambda p: Gate.Ry(p[0]), "Rz": lambda p: Gate.Rz(p[0]), "FSim": lambda p: Gate.FSim(p[0], p[1]),
+}
+
+
+def gate_from_spec(spec):
+    """gate_from_spec (tests/common/mod.rs:154-188)."""
+    if spec["name"] == "Custom":
+        m = np.array(spec["matrix_re"]) + 1j * np.array(spec["matrix_im"])
+        diag = all(i == j or abs(m[i, j]) < 1e-15 for i in range(len(m)) for j in range(len(m)))
+        return Gate.Custom(m, diag, spec.get("label", ""))
+    return NAMED[spec["name"]](spec["params"])
+
+
+def elements_from_case(case):
+    """circuit_from_case (tests/common/mod.rs:216-240)."""
+    els = []
+    for spec in case["gates"]:
+        g = gate_from_spec(spec)
+        if not spec["controls"]:
+            els.append(put(spec["targets"], g))
+        else:
+            cfg = spec["control_configs"] or [True] * len(spec["controls"])
+            els.append(PositionedGate(g, spec["targets"], spec["controls"], cfg))
+    return els
+
+
+def parse_state(pairs):
+    return np.array([complex(a, b) for a, b in pairs], dtype=np.complex128)
+
+
+def parse_matrix(rows):
+    return np.array([[complex(a, b) for a, b in row] for row in rows], dtype=np.complex128)
+
+
+def qft_with_swaps(n):
+    """QFT with the final bit-reversal SWAPs (examples/qft.rs:30-54)."""
+    els = []
+    for i in range(n):
+        els.append(put([i], Gate.H))
+        for j in range(1, n - i):
+            els.append(control([i + j], [i], Gate.Phase(2.0 * math.pi / float(1 << (j + 1)))))
+    for i in range(n // 2):
+        els.append(PositionedGate(Gate.SWAP, [i, n - 1 - i]))
+    return els
+
+
+def qft_no_swaps(n):
+    """easybuild::qft_circuit (easybuild.rs:55-70)."""
+    els = []
+    for i in range(n):
+        els.append(put([i], Gate.H))
+        for j in range(1, n - i):
+            els.append(control([i + j], [i], Gate.Phase(2.0 * math.pi / float(1 << (j + 1)))))
+    return els
+
+
+def brickwork(n, depth, seed):
+    """Random brickwork: per layer Rx, Ry, Rz (angles U[0, 2pi)) on every qubit, then
+    CNOT(2i -> 2i+1) in even layers and CNOT(2i+1 -> 2i+2) in odd layers."""
+    rng = np.random.default_rng(seed)
+    els = []
+    for layer in range(depth):
+        for q in range(n):
+            a = rng.uniform(0.0, 2.0 * math.pi, size=3)
+            els.append(put([q], Gate.Rx(a[0])))
+            els.append(put([q], Gate.Ry(a[1])))
+            els.append(put([q], Gate.Rz(a[2])))
+        start = 0 if layer % 2 == 0 else 1
+        for c in range(start, n - 1, 2):
+            els.append(control([c], [c + 1], Gate.X))
+    return els
+
+
+def noisy_brickwork(n, depth, seed, p=1e-3, gamma=1e-3):
+    """Brickwork on a density matrix: after every gate, on each touched qubit, a
+    Depolarizing(p) then AmplitudeDamping(gamma) channel."""
+    out = []
+    for e in brickwork(n, depth, seed):
+        out.append(e)
+        for q in list(e.target_locs) + list(e.control_locs):
+            out.append(channel([q], NoiseChannel.Depolarizing(1, p)))
+            out.append(channel([q], NoiseChannel.AmplitudeDamping(gamma, 0.0)))
+    return out
+
+
+def random_circuit(n, ngates, seed, max_targets=3, max_ctrls=2, custom_targets=(1, 2, 3)):
+    """Random gates of every dispatch class: named gates, controlled (active high and
+    low), diagonal and dense Custom matrices on 1..max_targets targets, SWAPs."""
+    rng = np.random.default_rng(seed)
+    kinds = ["X", "Y", "Z", "H", "S", "T", "SWAP", "Phase", "Rx", "Ry", "Rz", "SqrtX", "SqrtY", "SqrtW",
+             "ISWAP", "FSim", "CustomDense", "CustomDiag"]
+    els = []
+    for _ in range(ngates):
+        k = kinds[rng.integers(len(kinds))]
+        if k in ("SWAP", "ISWAP", "FSim"):
+            nt = 2
+        elif k in ("CustomDense", "CustomDiag"):
+            nt = int(rng.choice([t for t in custom_targets if t <= n]))
+        else:
+            nt = 1
+        nc = int(rng.integers(0, max_ctrls + 1)) if n - nt > 0 else 0
+        nc = min(nc, n - nt)
+        locs = rng.permutation(n)[: nt + nc]
+        tl, cl = [int(x) for x in locs[:nt]], [int(x) for x in locs[nt:]]
+        cfg = [bool(rng.integers(2)) for _ in cl]
+        th = rng.uniform(-math.pi, math.pi, size=2)
+        if k == "CustomDense":
+            d = 1 << nt
+            a = rng.standard_normal((d, d)) + 1j * rng.standard_normal((d, d))
+            q, _ = np.linalg.qr(a)
+            g = Gate.Custom(q, False)
+        elif k == "CustomDiag":
+            d = 1 << nt
+            g = Gate.Custom(np.diag(np.exp(1j * rng.uniform(-math.pi, math.pi, size=d))), True)
+        else:
+            g = NAMED[k](th)
+        els.append(PositionedGate(g, tl, cl, cfg))
+    return els

@@ -1,0 +1,132 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/yaohip.h declares, and its host-side logic (validation, gate matrices,
+Kraus operators) agrees with the oracle. No device call is made."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from fixture_circuits import NAMED, random_circuit
+from oracle import oracle as O
+from yaohip import _abi as A
+from yaohip.circuit import (Circuit, CircuitError, Gate, NoiseChannel, PositionedGate, channel, control, lower,
+                            put)
+from test_oracle import KRAUS_FIXTURES
+
+
+def declared_symbols():
+    with open(os.path.join(ROOT, "include", "yaohip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*\*?(yh_[a-z0-9_]+)\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = A.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 40
+    for s in syms:
+        assert hasattr(L, s), s
+    # the ctypes signature table covers the header exactly
+    assert sorted(n for n, _, _ in A.SIGNATURES) == syms
+
+
+def test_struct_layout_matches_header():
+    # int32 x 4 + 12 + 16 + 16 int32, 4 doubles, pointer, 4 int32
+    assert ctypes.sizeof(A.YhOp) == 4 * 4 + 4 * (12 + 16 + 16) + 8 * 4 + 8 + 4 * 4
+
+
+def test_gate_matrices_bitwise_equal_to_oracle():
+    L = A.lib()
+    for name, mk in NAMED.items():
+        for th in ([0.0], [0.7], [-2.5], [1e-10], [np.pi]):
+            g = mk(th + [0.3])
+            ol = lower([put(list(range(g.num_sites())), g)])
+            out = np.zeros(32)
+            dim = ctypes.c_int()
+            A.check(L.yh_gate_matrix(ctypes.byref(ol.arr[0]), A.dptr(out), ctypes.byref(dim)))
+            d = dim.value
+            mine = A.deinterleave(out[:2 * d * d]).reshape(d, d)
+            assert np.array_equal(mine, O.gate_matrix(ol.arr[0])), name
+
+
+@pytest.mark.parametrize("name", sorted(KRAUS_FIXTURES))
+def test_kraus_bitwise_equal_to_oracle(name):
+    from yaohip.register import channel_kraus
+    ks = channel_kraus(KRAUS_FIXTURES[name])
+    ref = O.kraus(lower([channel([0], KRAUS_FIXTURES[name])]))
+    assert len(ks) == len(ref)
+    for a, b in zip(ks, ref):
+        assert np.array_equal(a, b)
+
+
+def test_kraus_multi_qubit_depolarizing_and_custom():
+    from yaohip.register import channel_kraus
+    ch = NoiseChannel.Depolarizing(2, 0.2)
+    ks = channel_kraus(ch, [0, 1])
+    ref = O.kraus(lower([channel([0, 1], ch)]))
+    assert len(ks) == 16 and all(np.array_equal(a, b) for a, b in zip(ks, ref))
+    k0 = np.diag([1.0, np.sqrt(0.5)])
+    k1 = np.array([[0, np.sqrt(0.5)], [0, 0]])
+    ks = channel_kraus(NoiseChannel.Custom([k0, k1]))
+    assert np.array_equal(ks[0], k0) and np.array_equal(ks[1], k1)
+
+
+def test_channel_parameter_validation():
+    from yaohip.register import channel_kraus
+    for bad in [NoiseChannel.BitFlip(1.5), NoiseChannel.PauliChannel(0.5, 0.5, 0.5),
+                NoiseChannel.AmplitudeDamping(-0.1), NoiseChannel.ThermalRelaxation(1.0, 3.0, 1.0),
+                NoiseChannel.Reset(0.7, 0.6)]:
+        with pytest.raises(A.YhError) as e:
+            channel_kraus(bad)
+        assert e.value.status == 8
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_validation_agrees_with_oracle_on_random_circuits(seed):
+    n = 6
+    els = random_circuit(n, 30, seed)
+    ol = lower(els)
+    assert A.lib().yh_validate(n, ol.arr, ol.n) == O.validate(n, ol) == 0
+    rng = np.random.default_rng(seed)
+    for _ in range(50):
+        e = els[rng.integers(len(els))]
+        bad = PositionedGate(e.gate, list(e.target_locs), list(e.control_locs), list(e.control_configs))
+        kind = rng.integers(4)
+        if kind == 0:
+            bad.target_locs[0] = n + int(rng.integers(3))
+        elif kind == 1 and bad.control_locs:
+            bad.control_locs[0] = bad.target_locs[0]
+        elif kind == 2:
+            bad.control_configs = bad.control_configs + [True]
+        else:
+            bad.target_locs = bad.target_locs + [x for x in range(n) if x not in bad.target_locs
+                                                 and x not in bad.control_locs][:1]
+        ob = lower([bad])
+        assert A.lib().yh_validate(n, ob.arr, ob.n) == O.validate(n, ob)
+
+
+def test_circuit_error_variants():
+    with pytest.raises(CircuitError) as e:
+        Circuit.qubits(2, [PositionedGate(Gate.X, [1], [0], [True, True])])
+    assert e.value.variant == "ControlConfigLengthMismatch"
+    with pytest.raises(CircuitError) as e:
+        Circuit.qubits(2, [put([5], Gate.H)])
+    assert e.value.variant == "LocOutOfRange"
+    with pytest.raises(CircuitError) as e:
+        Circuit.qubits(2, [control([0], [0], Gate.X)])
+    assert e.value.variant == "OverlappingLocs"
+    with pytest.raises(CircuitError) as e:
+        Circuit.qubits(2, [put([0, 1], Gate.H)])
+    assert e.value.variant == "MatrixSizeMismatch"
+
+
+def test_no_device_here_fails_loudly():
+    """Without a GPU the library reports NO_DEVICE instead of computing on the CPU."""
+    if os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK):
+        pytest.skip("a GPU is visible")
+    h = ctypes.c_void_p()
+    st = A.lib().yh_reg_create(4, A.C128, ctypes.byref(h))
+    assert st == 14

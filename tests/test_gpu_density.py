@@ -1,0 +1,130 @@
+"""GPU parity of the density-matrix path (gates, Kraus channels, trace, purity,
+expect_dm, probs) against the oracle and the reference's noise fixtures.
+Bar: entries within 1e-12 relative; trace preserved to 1e-12."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err, seeded_state
+from fixture_circuits import noisy_brickwork, random_circuit
+from oracle import oracle as O
+from test_oracle import KRAUS_FIXTURES
+from yaohip.circuit import (Circuit, Gate, NoiseChannel, OperatorPolynomial, OperatorString, Op, channel, control,
+                            lower, put)
+
+pytestmark = pytest.mark.gpu
+
+
+def _mat(d):
+    return np.array(d["re"]) + 1j * np.array(d["im"])
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_dm_cases_match_julia(gpu, fused):
+    fx = load_golden("noise.json")["dm_cases"]
+    cases = {
+        "h_phaseflip_0.1": (1, [put([0], Gate.H), channel([0], NoiseChannel.PhaseFlip(0.1))]),
+        "bitflip_0.1_on_zero": (1, [channel([0], NoiseChannel.BitFlip(0.1))]),
+        "h_depolarizing_0.1": (1, [put([0], Gate.H), channel([0], NoiseChannel.Depolarizing(1, 0.1))]),
+        "bell_depolarizing_q0_0.1": (2, [put([0], Gate.H), control([0], [1], Gate.X),
+                                         channel([0], NoiseChannel.Depolarizing(1, 0.1))]),
+        "h_amplitude_damping_0.3": (1, [put([0], Gate.H), channel([0], NoiseChannel.AmplitudeDamping(0.3, 0.0))]),
+    }
+    for name, (n, els) in cases.items():
+        dm = gpu.DensityMatrix.zero_state(n)
+        dm.apply(Circuit.qubits(n, els), fused=fused)
+        assert np.max(np.abs(dm.state_data() - _mat(fx[name]["rho"]).reshape(-1))) < 1e-10, name
+
+
+def test_dm_expectation_fixtures(gpu):
+    fx = load_golden("noise.json")["expectations"]
+    cases = {
+        "z_h_pure": ([put([0], Gate.H)], Op.Z),
+        "x_h_depolarizing_0.1": ([put([0], Gate.H), channel([0], NoiseChannel.Depolarizing(1, 0.1))], Op.X),
+        "z_h_phaseflip_0.1": ([put([0], Gate.H), channel([0], NoiseChannel.PhaseFlip(0.1))], Op.Z),
+    }
+    for name, (els, op) in cases.items():
+        dm = gpu.DensityMatrix.zero_state(1)
+        dm.apply(Circuit.qubits(1, els))
+        v = gpu.expect_dm(dm, OperatorPolynomial.single(0, op, 1.0))
+        assert abs(v - complex(fx[name]["value_re"], fx[name]["value_im"])) < 1e-10, name
+
+
+def random_noisy(n, ngates, seed):
+    rng = np.random.default_rng(seed)
+    chans = list(KRAUS_FIXTURES.values()) + [NoiseChannel.Reset(0.1, 0.2), NoiseChannel.Reset(0.3, 0.0)]
+    out = []
+    for e in random_circuit(n, ngates, seed, max_ctrls=1, custom_targets=(1, 2, 3)):
+        out.append(e)
+        if rng.random() < 0.6:
+            q = int(rng.integers(n))
+            out.append(channel([q], chans[rng.integers(len(chans))]))
+        if rng.random() < 0.1 and n >= 2:
+            a, b = [int(x) for x in rng.permutation(n)[:2]]
+            out.append(channel([a, b], NoiseChannel.Depolarizing(2, 0.05)))
+    return out
+
+
+@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("n,seed", [(1, 0), (3, 1), (5, 2), (7, 3)])
+def test_random_noisy_circuits_vs_oracle(gpu, n, seed, fused):
+    els = random_noisy(n, 40, 100 + seed)
+    psi = seeded_state(n, seed)
+    rho0 = O.dm_from_reg(n, psi)
+    want = O.dm_apply(n, rho0, lower(els))
+    dm = gpu.DensityMatrix.from_reg(gpu.ArrayReg.from_vec(n, psi))
+    assert rel_err(dm.state_data(), rho0) < 1e-15
+    dm.apply(Circuit.qubits(n, els), fused=fused)
+    got = dm.state_data()
+    assert rel_err(got, want) < 1e-12
+    assert abs(dm.trace() - O.dm_trace(n, want)) < 1e-12
+    assert abs(dm.trace() - 1.0) < 1e-12
+    assert abs(dm.purity() - O.dm_purity(n, want)) < 1e-12
+    H = OperatorPolynomial([1.0, 0.5j, -0.3], [OperatorString([(0, Op.Z)] + ([(1, Op.X)] if n > 1 else [])),
+                                              OperatorString([(n - 1, Op.Y)]), OperatorString([(0, Op.Pu)])])
+    assert abs(gpu.expect_dm(dm, H) - O.dm_expect(n, want, H)) < 1e-12
+    assert np.array_equal(gpu.probs(dm), O.dm_probs(n, want))
+    locs = [n - 1, 0] if n > 1 else [0]
+    assert np.array_equal(gpu.probs(dm, locs), O.dm_probs(n, want, locs))
+
+
+@pytest.mark.parametrize("fused", [False, True])
+def test_noisy_brickwork_8q(gpu, fused):
+    """The noisy brickwork workload at 8 qubits, depth 3."""
+    n = 8
+    els = noisy_brickwork(n, 3, 17)
+    z = np.zeros(1 << n, dtype=complex)
+    z[0] = 1
+    want = O.dm_apply(n, O.dm_from_reg(n, z), lower(els))
+    dm = gpu.DensityMatrix.zero_state(n)
+    dm.apply(Circuit.qubits(n, els), fused=fused)
+    assert rel_err(dm.state_data(), want) < 1e-12
+    assert abs(dm.trace() - 1) < 1e-12
+    for i in range(n - 1):
+        zz = OperatorPolynomial([1.0], [OperatorString([(i, Op.Z), (i + 1, Op.Z)])])
+        assert abs(gpu.expect_dm(dm, zz) - O.dm_expect(n, want, zz)) < 1e-12
+
+
+def test_dm_unitary_matches_pure(gpu):
+    n = 6
+    psi = seeded_state(n, 44)
+    els = random_circuit(n, 50, 44, max_ctrls=2, custom_targets=(1, 2, 3, 4))
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    dm = gpu.DensityMatrix.from_reg(reg)
+    dm.apply(Circuit.qubits(n, els))
+    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    out = reg.state_vec()
+    assert rel_err(dm.state_data(), np.outer(out, out.conj()).reshape(-1)) < 1e-12
+    assert abs(dm.purity() - 1) < 1e-12
+
+
+def test_dm_large_trace_and_purity_14q(gpu):
+    """14 qubits (2^28 entries): trace kept at 1, purity 1 for a unitary layer, and
+    below 1 after noise."""
+    n = 14
+    dm = gpu.DensityMatrix.zero_state(n)
+    dm.apply(Circuit.qubits(n, noisy_brickwork(n, 1, 3)[:60]), fused=True)
+    assert abs(dm.trace() - 1) < 1e-12
+    p = dm.purity()
+    assert 0.9 < p < 1.0

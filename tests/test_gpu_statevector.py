@@ -1,0 +1,251 @@
+"""GPU parity of the state-vector path against the oracle and the reference's golden
+vectors. Bars: amplitudes within 1e-12 relative (max-abs error / max-abs amplitude)
+in complex128; probabilities of the full distribution and sampled outcomes
+bit-exact; complex64 within 2e-5 relative (its error is printed)."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import load_golden, rel_err, seeded_state
+from fixture_circuits import (NAMED, brickwork, elements_from_case, parse_matrix, parse_state, qft_with_swaps,
+                              random_circuit)
+from oracle import oracle as O
+from yaohip.circuit import (Circuit, Gate, OperatorPolynomial, OperatorString, Op, PositionedGate, control, lower,
+                            put, tfim_hamiltonian)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def run_gpu(gpu, n, psi, els, dtype="c128", fused=0):
+    reg = gpu.ArrayReg.from_vec(n, psi, dtype)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg, fused)
+    return reg.state_vec()
+
+
+def test_instruct_fixtures(gpu):
+    tested = 0
+    for case in load_golden("instruct.json")["cases"]:
+        label = case["label"]
+        if "locs" not in case or "regression" in label:
+            continue
+        n = case["nbits"]
+        cl = case.get("ctrl_locs", [])
+        cb = [bool(b) for b in case.get("ctrl_bits", [])]
+        if "gate_matrix" in case:
+            m = parse_matrix(case["gate_matrix"])
+            gates = [Gate.Custom(m, False)]
+            if np.count_nonzero(m - np.diag(np.diag(m))) == 0:
+                gates.append(Gate.Custom(m, True))   # the diagonal kernel as well
+            elss = [[PositionedGate(g, case["locs"], cl, cb)] for g in gates]
+        elif case["gate_name"] in NAMED:
+            g = NAMED[case["gate_name"]]([case.get("theta", 0.0)])
+            if g.num_sites() == 2:
+                elss = [[PositionedGate(g, case["locs"], cl, cb)]]
+            else:
+                elss = [[PositionedGate(g, [l], cl, cb) for l in case["locs"]]]
+        else:
+            continue
+        inp, want = parse_state(case["input_state"]), parse_state(case["output_state"])
+        for els in elss:
+            got = run_gpu(gpu, n, inp, els)
+            assert np.max(np.abs(got - want)) < 1e-10, label
+            assert rel_err(got, O.apply(n, inp, lower(els))) < TOL, label
+        tested += 1
+    assert tested >= 40
+
+
+def test_instruct_regression_fixture(gpu):
+    case = [c for c in load_golden("instruct.json")["cases"] if "regression" in c["label"]][0]
+    m = parse_matrix(case["gate_matrix"])
+    els = [put(pair, Gate.Custom(m, False)) for pair in case["gate_pairs"]]
+    got = run_gpu(gpu, case["nbits"], parse_state(case["input_state"]), els)
+    assert np.max(np.abs(got - parse_state(case["output_state"]))) < 1e-8
+
+
+def test_apply_fixtures(gpu):
+    tested = 0
+    for case in load_golden("apply.json")["cases"]:
+        if any(d != 2 for d in case["dims"]):
+            continue
+        n = len(case["dims"])
+        if case.get("input_state_re") is not None:
+            inp = np.array(case["input_state_re"]) + 1j * np.array(case["input_state_im"])
+        else:
+            inp = np.zeros(1 << n, dtype=complex)
+            inp[0] = 1
+        els = elements_from_case(case)
+        got = run_gpu(gpu, n, inp, els)
+        want = np.array(case["output_state_re"]) + 1j * np.array(case["output_state_im"])
+        assert np.max(np.abs(got - want)) < 1e-10, case["label"]
+        assert rel_err(got, O.apply(n, inp, lower(els))) < TOL, case["label"]
+        tested += 1
+    assert tested >= 60
+
+
+@pytest.mark.parametrize("n,seed", [(1, 0), (2, 1), (3, 2), (7, 3), (10, 4), (12, 5), (13, 6)])
+def test_random_circuits_all_dispatch_classes(gpu, n, seed):
+    custom = (1, 2, 3, 4, 5, 6, 7) if n >= 7 else tuple(range(1, n + 1))
+    els = random_circuit(n, 160, 500 + seed, max_ctrls=min(3, n - 1), custom_targets=custom)
+    psi = seeded_state(n, seed)
+    print(f"[case] random circuit n={n} seed={seed}", flush=True)
+    got = run_gpu(gpu, n, psi, els)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+@pytest.mark.parametrize("n", [6, 11])
+def test_complex64_error_vs_fp64_oracle(gpu, n):
+    els = random_circuit(n, 120, 900 + n, max_ctrls=2, custom_targets=(1, 2, 3))
+    psi = seeded_state(n, n)
+    got = run_gpu(gpu, n, psi, els, "c64")
+    err = rel_err(got, O.apply(n, psi, lower(els)))
+    print(f"complex64 relative error n={n}: {err:.3e}")
+    assert err < 2e-5
+
+
+def test_brickwork_20q_vs_oracle(gpu):
+    n = 20
+    els = brickwork(n, 4, 31)
+    psi = seeded_state(n, 31)
+    got = run_gpu(gpu, n, psi, els)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+def dagger(els):
+    inv = {"Rx": Gate.Rx, "Ry": Gate.Ry, "Rz": Gate.Rz, "Phase": Gate.Phase}
+    out = []
+    for e in reversed(els):
+        g = e.gate
+        from yaohip import _abi as A
+        name = A.GATE_NAMES[g.kind]
+        g2 = inv[name](-g.params[0]) if name in inv else g
+        out.append(PositionedGate(g2, e.target_locs, e.control_locs, e.control_configs))
+    return out
+
+
+def test_brickwork_26q_round_trip_and_norm(gpu):
+    """Full-size property: U then U^dagger returns the input; the norm is kept."""
+    n = 26
+    els = brickwork(n, 2, 5)
+    psi = seeded_state(n, 9)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    assert abs(reg.norm() - 1.0) < 1e-12
+    gpu.apply_inplace(Circuit.qubits(n, dagger(els)), reg)
+    assert rel_err(reg.state_vec(), psi) < 1e-12
+
+
+def test_norm_normalize_fidelity(gpu):
+    n = 9
+    psi = seeded_state(n, 1) * 3.0
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    assert abs(reg.norm() - O.norm(n, psi)) < 1e-13
+    other = gpu.ArrayReg.from_vec(n, seeded_state(n, 2))
+    assert abs(reg.fidelity(other) - O.fidelity(n, psi, seeded_state(n, 2))) < 1e-12
+    reg.normalize()
+    assert rel_err(reg.state_vec(), psi / 3.0) < 1e-15
+    z = gpu.ArrayReg.zero_state(5)
+    s = z.state_vec()
+    assert s[0] == 1 and np.count_nonzero(s) == 1
+
+
+def test_expectations_vs_oracle(gpu):
+    n = 11
+    psi = seeded_state(n, 4)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    H = tfim_hamiltonian(n, 1.0, 1.0)
+    assert abs(gpu.expect_arrayreg(reg, H) - O.expect(n, psi, H)) < 1e-12
+    rng = np.random.default_rng(3)
+    coeffs, strings = [], []
+    for _ in range(21):
+        k = int(rng.integers(1, 5))
+        sites = rng.permutation(n)[:k]
+        strings.append(OperatorString([(int(s), int(rng.integers(0, 8))) for s in sites]))
+        coeffs.append(complex(rng.standard_normal(), rng.standard_normal()))
+    P = OperatorPolynomial(coeffs, strings)
+    assert abs(gpu.expect_arrayreg(reg, P) - O.expect(n, psi, P)) < 1e-12
+
+
+def test_probs_bit_exact(gpu):
+    n = 14
+    psi = seeded_state(n, 8)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    assert np.array_equal(gpu.probs(reg), O.probs(n, psi))
+    for locs in ([0], [3, 1], [13, 0, 7], list(range(n))[::-1], [5, 9, 2, 11]):
+        assert np.array_equal(gpu.probs(reg, locs), O.probs(n, psi, locs)), locs
+
+
+def test_probs_chunked_marginal_close(gpu):
+    """Beyond 2^18 members per bucket the sum is split into ordered chunks."""
+    n = 21
+    psi = seeded_state(n, 12)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    got = gpu.probs(reg, [4, 17])
+    assert rel_err(got, O.probs(n, psi, [4, 17])) < 1e-13
+
+
+def test_sample_bit_exact_with_same_uniforms(gpu):
+    n = 12
+    psi = seeded_state(n, 21)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    rng = np.random.default_rng(42)
+    u = rng.random(4096)
+    # adversarial uniforms: exactly at the sequential partial sums
+    cs, acc = [], 0.0
+    for p in O.probs(n, psi):
+        acc += p
+        cs.append(acc)
+    u = np.concatenate([u, np.array(cs[::97]), [0.0, 1.0 - 2 ** -53, cs[-1]]])
+    for locs in (None, [0, 5], [11, 3, 7]):
+        got = gpu.sample(reg, locs, u)
+        want = O.sample(n, psi, locs, u)
+        assert np.array_equal(got, want), locs
+
+
+@pytest.mark.parametrize("post", [0, 1, 2])
+def test_measure_postprocess_vs_oracle(gpu, post):
+    n = 8
+    psi = seeded_state(n, 30 + post)
+    locs = [6, 1, 3]
+    for r in (0.05, 0.5, 0.93):
+        reg = gpu.ArrayReg.from_vec(n, psi)
+        P = [gpu.PostProcess.NoPostProcess, gpu.PostProcess.ResetTo(5), gpu.PostProcess.RemoveMeasured][post]
+        res = gpu.measure_with_postprocess(reg, locs, P, lambda: r)
+        bits_o, state_o = O.measure(n, psi, locs, post, 5, r)
+        if post == 2:
+            bits, newreg = res
+            assert rel_err(newreg.state_vec(), state_o) < 1e-12
+        else:
+            bits = res
+            assert rel_err(reg.state_vec(), state_o) < 1e-12
+        assert bits == bits_o
+
+
+def test_measure_errors(gpu):
+    reg = gpu.ArrayReg.from_vec(3, seeded_state(3, 0))
+    with pytest.raises(gpu.YhError) as e:
+        gpu.measure_with_postprocess(reg, [0, 0], gpu.PostProcess.NoPostProcess, lambda: 0.5)
+    assert e.value.status == 7
+    with pytest.raises(gpu.YhError) as e:
+        gpu.measure_with_postprocess(reg, [0], gpu.PostProcess.ResetTo(2), lambda: 0.5)
+    assert e.value.status == 13
+    with pytest.raises(gpu.YhError) as e:
+        gpu.apply_inplace(Circuit.qubits(4, [put([0], Gate.H)]), reg)
+    assert e.value.status == 6
+
+
+def test_qft12_config(gpu):
+    """12-qubit QFT with SWAPs on a seeded N(0,1) state: amplitudes, Z(0)Z(1), 1024 shots."""
+    n = 12
+    psi = seeded_state(n, 2024)
+    els = qft_with_swaps(n)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    want = O.apply(n, psi, lower(els))
+    assert rel_err(reg.state_vec(), want) < TOL
+    zz = OperatorPolynomial([1.0], [OperatorString([(0, Op.Z), (1, Op.Z)])])
+    assert abs(gpu.expect_arrayreg(reg, zz) - O.expect(n, want, zz)) < 1e-12
+    u = np.random.default_rng(1024).random(1024)
+    assert np.array_equal(gpu.sample(reg, None, u), O.sample(n, reg.state_vec(), None, u))

@@ -1,0 +1,72 @@
+// runtime.h — device buffers and the per-handle workspace (one HIP stream each).
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+
+namespace yh {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  int ensure(size_t b);  // grows (contents are not preserved)
+  void release();
+};
+
+struct PinBuf {  // page-locked host staging (hipHostMalloc)
+  void* p = nullptr;
+  size_t bytes = 0;
+  PinBuf() = default;
+  PinBuf(const PinBuf&) = delete;
+  PinBuf& operator=(const PinBuf&) = delete;
+  ~PinBuf();
+  int ensure(size_t b);
+};
+
+struct Workspace {
+  hipStream_t stream = nullptr;
+  DevBuf red;        // reduction partials followed by results
+  DevBuf mats;       // per-call operand arena (matrices, offset tables)
+  DevBuf scratch;    // out-of-place copies
+  DevBuf terms;      // Pauli terms in device form
+  DevBuf locs;       // measured loc bits
+  DevBuf ws;         // chunked probability sums
+  DevBuf probs;      // probabilities and their CDF
+  DevBuf shots;      // caller-supplied uniforms
+  DevBuf shots_out;  // sampled outcomes
+  PinBuf pin;
+  // launch timing (measurement hooks): events around each launch while enabled
+  struct TimedLaunch {
+    int kclass;
+    hipEvent_t a, b;
+  };
+  bool timing = false;
+  std::vector<TimedLaunch> pending;
+  uint64_t t_count[YH_K_NCLASS] = {};
+  double t_ms[YH_K_NCLASS] = {};
+  int time_begin(int kclass);  // call before a launch
+  int time_end();              // call after it
+  int time_collect();          // after the stream has drained
+  Workspace() = default;
+  Workspace(const Workspace&) = delete;
+  Workspace& operator=(const Workspace&) = delete;
+  ~Workspace();
+  int init();
+  double* partial();
+  double* result();
+  int sync();
+  int upload(const void* host, size_t bytes, DevBuf& dst);
+  int fetch(void* host, const void* dev, size_t bytes);
+};
+
+int ensure_init();
+
+template <class R>
+int apply_gates(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
+
+}  // namespace yh

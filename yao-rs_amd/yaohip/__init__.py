@@ -1,0 +1,20 @@
+"""yaohip — MI355X (gfx950) backend for yao-rs's dense simulation path.
+
+The names mirror yao-rs's public API (src/lib.rs:25-48). All simulation runs in
+libyaohip.so through the C ABI of include/yaohip.h.
+"""
+from . import _abi
+from .circuit import (Annotation, Circuit, CircuitError, Gate, NoiseChannel, Op, OperatorPolynomial,
+                      OperatorString, PositionedChannel, PositionedGate, channel, control, label, lower,
+                      put, tfim_hamiltonian)
+from .register import (ArrayReg, DensityMatrix, PostProcess, apply, apply_inplace, channel_kraus,
+                       expect_arrayreg, expect_dm, gate_matrix, measure_with_postprocess, probs, sample)
+
+YhError = _abi.YhError
+
+__all__ = [
+    "Annotation", "ArrayReg", "Circuit", "CircuitError", "DensityMatrix", "Gate", "NoiseChannel", "Op",
+    "OperatorPolynomial", "OperatorString", "PositionedChannel", "PositionedGate", "PostProcess", "YhError",
+    "apply", "apply_inplace", "channel", "channel_kraus", "control", "expect_arrayreg", "expect_dm",
+    "gate_matrix", "label", "lower", "measure_with_postprocess", "probs", "put", "sample", "tfim_hamiltonian",
+]

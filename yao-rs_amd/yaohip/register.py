@@ -1,0 +1,271 @@
+"""Device-resident registers and the simulation entry points.
+
+Mirrors yao-rs's public functions over the C ABI: `ArrayReg` (src/register.rs),
+`apply` / `apply_inplace` (src/apply.rs:123-154), `expect_arrayreg` / `expect_dm`
+(src/expect.rs:7,38), `probs` / `measure_with_postprocess` (src/measure.rs:82,243),
+`DensityMatrix` (src/density_matrix.rs). Every amplitude-touching call runs on the
+GPU through libyaohip.so; there is no host fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _abi as A
+from .circuit import Circuit, OperatorPolynomial
+
+
+def _lib():
+    A.init()
+    return A.lib()
+
+
+class ArrayReg:
+    """ArrayReg on the device (register.rs:15-18). dtype: "c128" or "c64"."""
+
+    def __init__(self, handle, dtype):
+        self._h = handle
+        self.dtype = dtype
+
+    @staticmethod
+    def _new(nbits, dtype="c128"):
+        h = ctypes.c_void_p()
+        A.check(_lib().yh_reg_create(nbits, A.C128 if dtype == "c128" else A.C64, ctypes.byref(h)))
+        return ArrayReg(h, dtype)
+
+    @staticmethod
+    def zero_state(nbits, dtype="c128"):
+        """zero_state (register.rs:21-26)."""
+        return ArrayReg._new(nbits, dtype)
+
+    @staticmethod
+    def from_vec(nbits, state, dtype="c128"):
+        """from_vec (register.rs:65-69)."""
+        r = ArrayReg._new(nbits, dtype)
+        buf = A.interleave(state)
+        if buf.size != 2 << nbits:
+            raise ValueError("state length must be 2^nbits")
+        A.check(A.lib().yh_reg_upload(r._h, A.dptr(buf), 1 << nbits))
+        return r
+
+    @staticmethod
+    def uniform_state(nbits, dtype="c128"):
+        """uniform_state (register.rs:47-54)."""
+        amp = 1.0 / np.sqrt(float(1 << nbits))
+        return ArrayReg.from_vec(nbits, np.full(1 << nbits, amp, dtype=np.complex128), dtype)
+
+    @staticmethod
+    def ghz_state(nbits, dtype="c128"):
+        """ghz_state (register.rs:56-63)."""
+        s = np.zeros(1 << nbits, dtype=np.complex128)
+        s[0] = s[-1] = 1.0 / np.sqrt(2.0)
+        return ArrayReg.from_vec(nbits, s, dtype)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and A._lib is not None:
+            A._lib.yh_reg_destroy(h)
+            self._h = None
+
+    def nqubits(self):
+        n = ctypes.c_int()
+        A.check(A.lib().yh_reg_nbits(self._h, ctypes.byref(n)))
+        return n.value
+
+    nbits = property(nqubits)
+
+    def state_vec(self):
+        """state_vec (register.rs:91-93), copied to the host as complex128."""
+        n = self.nqubits()
+        buf = np.zeros(2 << n)
+        A.check(A.lib().yh_reg_download(self._h, A.dptr(buf), 1 << n))
+        return A.deinterleave(buf)
+
+    def clone(self):
+        h = ctypes.c_void_p()
+        A.check(A.lib().yh_reg_clone(self._h, ctypes.byref(h)))
+        return ArrayReg(h, self.dtype)
+
+    def norm(self):
+        v = ctypes.c_double()
+        A.check(A.lib().yh_reg_norm(self._h, ctypes.byref(v)))
+        return v.value
+
+    def normalize(self):
+        A.check(A.lib().yh_reg_normalize(self._h))
+
+    def fidelity(self, other):
+        v = ctypes.c_double()
+        A.check(A.lib().yh_reg_fidelity(self._h, other._h, ctypes.byref(v)))
+        return v.value
+
+    def apply(self, circuit):
+        """Register::apply (register.rs:127-139)."""
+        apply_inplace(circuit, self)
+
+
+def apply_inplace(circuit, reg, fused=0):
+    """apply_inplace (apply.rs:123-147). fused=k>0 packs gate runs on up to k qubits."""
+    ops = circuit.ops
+    if fused:
+        A.check(A.lib().yh_apply_fused(reg._h, circuit.nbits, ops.arr, ops.n, int(fused)))
+    else:
+        A.check(A.lib().yh_apply(reg._h, circuit.nbits, ops.arr, ops.n))
+
+
+def apply(circuit, reg, fused=0):
+    """apply (apply.rs:150-154): returns a new register."""
+    out = reg.clone()
+    apply_inplace(circuit, out, fused)
+    return out
+
+
+def expect_arrayreg(reg, op: OperatorPolynomial):
+    """expect_arrayreg (expect.rs:7-11)."""
+    T, coeffs, tl, sites, ops = op.flat()
+    out = np.zeros(2)
+    A.check(A.lib().yh_expect(reg._h, T, A.dptr(coeffs), A.i32ptr(tl), A.i32ptr(sites), A.i32ptr(ops),
+                              A.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def probs(state, locs=None):
+    """probs (measure.rs:82-87) for an ArrayReg or DensityMatrix."""
+    n = state.nqubits()
+    m = (1 << n) if locs is None else (1 << len(locs))
+    out = np.zeros(m)
+    la = None if locs is None else np.array(list(locs) or [0], dtype=np.int32)
+    lp = None if la is None else A.i32ptr(la)
+    nl = 0 if locs is None else len(locs)
+    if isinstance(state, DensityMatrix):
+        A.check(A.lib().yh_dm_probs(state._h, lp, nl, A.dptr(out)))
+    else:
+        A.check(A.lib().yh_probs(state._h, lp, nl, A.dptr(out)))
+    return out
+
+
+def sample(reg, locs, uniforms):
+    """One outcome index per uniform (sample_from_probs, measure.rs:140-150)."""
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    out = np.zeros(len(u), dtype=np.uint64)
+    la = None if locs is None else np.array(list(locs) or [0], dtype=np.int32)
+    A.check(A.lib().yh_sample(reg._h, None if la is None else A.i32ptr(la), 0 if locs is None else len(locs),
+                              A.dptr(u), len(u), A.u64ptr(out)))
+    return out
+
+
+class PostProcess:
+    """PostProcess (measure.rs:18-23)."""
+    NoPostProcess = ("none", 0)
+
+    @staticmethod
+    def ResetTo(v):
+        return ("reset", int(v))
+
+    RemoveMeasured = ("remove", 0)
+
+
+def measure_with_postprocess(reg, locs, post, rng):
+    """measure_with_postprocess (measure.rs:243-274). `rng` is a callable returning
+    the next f64 uniform in [0, 1) (the caller owns the stream, as `&mut impl Rng`).
+    Returns the outcome bits; with RemoveMeasured returns (bits, new register)."""
+    kind, val = post
+    code = {"none": 0, "reset": 1, "remove": 2}[kind]
+    bits = np.zeros(len(locs), dtype=np.int32)
+    target = reg.clone() if code == 2 else reg
+    A.check(A.lib().yh_measure(target._h, A.i32ptr(np.array(locs, dtype=np.int32)), len(locs), code, val,
+                               float(rng()), A.i32ptr(bits)))
+    out = [int(b) for b in bits]
+    return (out, target) if code == 2 else out
+
+
+class DensityMatrix:
+    """DensityMatrix on the device (density_matrix.rs:9-12), complex128."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    @staticmethod
+    def zero_state(nbits):
+        h = ctypes.c_void_p()
+        A.check(_lib().yh_dm_create(nbits, ctypes.byref(h)))
+        return DensityMatrix(h)
+
+    @staticmethod
+    def from_reg(reg):
+        h = ctypes.c_void_p()
+        A.check(A.lib().yh_dm_from_reg(reg._h, ctypes.byref(h)))
+        return DensityMatrix(h)
+
+    @staticmethod
+    def from_matrix(nbits, rho):
+        d = DensityMatrix.zero_state(nbits)
+        buf = A.interleave(np.asarray(rho).reshape(-1))
+        A.check(A.lib().yh_dm_upload(d._h, A.dptr(buf), 1 << (2 * nbits)))
+        return d
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and A._lib is not None:
+            A._lib.yh_dm_destroy(h)
+            self._h = None
+
+    def nbits(self):
+        n = ctypes.c_int()
+        A.check(A.lib().yh_dm_nbits(self._h, ctypes.byref(n)))
+        return n.value
+
+    nqubits = nbits
+
+    def state_data(self):
+        n = self.nbits()
+        buf = np.zeros(2 << (2 * n))
+        A.check(A.lib().yh_dm_download(self._h, A.dptr(buf), 1 << (2 * n)))
+        return A.deinterleave(buf)
+
+    def apply(self, circuit: Circuit, fused=False):
+        """Register::apply for DensityMatrix (density_matrix.rs:244-259)."""
+        ops = circuit.ops
+        fn = A.lib().yh_dm_apply_fused if fused else A.lib().yh_dm_apply
+        A.check(fn(self._h, circuit.nbits, ops.arr, ops.n))
+
+    def trace(self):
+        out = np.zeros(2)
+        A.check(A.lib().yh_dm_trace(self._h, A.dptr(out)))
+        return complex(out[0], out[1])
+
+    def purity(self):
+        v = ctypes.c_double()
+        A.check(A.lib().yh_dm_purity(self._h, ctypes.byref(v)))
+        return v.value
+
+
+def expect_dm(dm, op: OperatorPolynomial):
+    """expect_dm (expect.rs:38-42); sites are LSB-first bit positions."""
+    T, coeffs, tl, sites, ops = op.flat()
+    out = np.zeros(2)
+    A.check(A.lib().yh_dm_expect(dm._h, T, A.dptr(coeffs), A.i32ptr(tl), A.i32ptr(sites), A.i32ptr(ops),
+                                 A.dptr(out)))
+    return complex(out[0], out[1])
+
+
+def channel_kraus(noise, locs=None):
+    """NoiseChannel::kraus_operators (noise.rs:183) computed by the backend's host side."""
+    from .circuit import channel, lower
+    n = noise.num_qubits()
+    ol = lower([channel(list(range(n)) if locs is None else locs, noise)])
+    out = np.zeros(2 * 256 * 16)
+    nops, dim = ctypes.c_int(), ctypes.c_int()
+    A.check(A.lib().yh_channel_kraus(ctypes.byref(ol.arr[0]), A.dptr(out), 256, ctypes.byref(nops),
+                                     ctypes.byref(dim)))
+    d = dim.value
+    return [A.deinterleave(out[2 * d * d * i: 2 * d * d * (i + 1)]).reshape(d, d) for i in range(nops.value)]
+
+
+def gate_matrix(gate):
+    from .circuit import put, lower
+    ol = lower([put(list(range(gate.num_sites())), gate)])
+    out = np.zeros(2 * 16 * 16 * 4)
+    dim = ctypes.c_int()
+    A.check(A.lib().yh_gate_matrix(ctypes.byref(ol.arr[0]), A.dptr(out), ctypes.byref(dim)))
+    d = dim.value
+    return A.deinterleave(out[:2 * d * d]).reshape(d, d)
