@@ -100,8 +100,10 @@ def random_circuit(n, ngates, seed, max_targets=3, max_ctrls=2, custom_targets=(
     """Random gates of every dispatch class: named gates, controlled (active high and
     low), diagonal and dense Custom matrices on 1..max_targets targets, SWAPs."""
     rng = np.random.default_rng(seed)
-    kinds = ["X", "Y", "Z", "H", "S", "T", "SWAP", "Phase", "Rx", "Ry", "Rz", "SqrtX", "SqrtY", "SqrtW",
-             "ISWAP", "FSim", "CustomDense", "CustomDiag"]
+    kinds = ["X", "Y", "Z", "H", "S", "T", "Phase", "Rx", "Ry", "Rz", "SqrtX", "SqrtY", "SqrtW",
+             "CustomDense", "CustomDiag"]
+    if n >= 2:
+        kinds += ["SWAP", "ISWAP", "FSim"]
     els = []
     for _ in range(ngates):
         k = kinds[rng.integers(len(kinds))]

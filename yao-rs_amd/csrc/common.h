@@ -53,6 +53,15 @@ __device__ __forceinline__ V cmul2(V a, V w, V b, V v) {
   return cadd(cmul(a, w), cmul(b, v));
 }
 
+// |z|^2 = re*re + im*im rounded after each operation, as num-complex's norm_sqr in
+// Rust (which never fuses); contraction into an FMA is switched off for this scope.
+__device__ __forceinline__ double norm_sqr_exact(double re, double im) {
+#pragma clang fp contract(off)
+  const double a = re * re;
+  const double b = im * im;
+  return a + b;
+}
+
 // Insert a zero bit at each of the npos sorted (ascending) positions of `pos`.
 // Maps a free index k to the basis index with those bits locked to 0
 // (bitbasis itercontrol / group_shift, iterate_control.rs:82-142).

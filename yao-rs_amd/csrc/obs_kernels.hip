@@ -185,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_probs_full(const typename Cplx<R>::V
   uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   const double re = s[i].x, im = s[i].y;
-  out[i] = __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
+  out[i] = norm_sqr_exact(re, im);
 }
 
 __device__ __forceinline__ uint64_t bucket_base(uint64_t m, const int32_t* locbits, int nlocs) {
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_probs_marginal(const typename Cplx<R
       p = fmax(double(s[b * dim + b].x), 0.0);
     } else {
       const double re = s[b].x, im = s[b].y;
-      p = __dadd_rn(__dmul_rn(re, re), __dmul_rn(im, im));
+      p = norm_sqr_exact(re, im);
     }
     acc = __dadd_rn(acc, p);
   }
