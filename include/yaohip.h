@@ -136,6 +136,10 @@ int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
  * one 2^k x 2^k block, applied with MFMA). max_fuse_qubits in [1,5]; 0 = default. */
 int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
                    int max_fuse_qubits);
+/* Host-only plan of yh_apply_fused for a circuit on nbits qubits: out[0] HBM passes
+ * of the tile kernel, out[1] gate groups, out[2] gates placed in tiles, out[3]
+ * gates run as their own launch. No device call. */
+int yh_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[4]);
 /* dispatch_arrayreg_gate for one gate element (apply.rs:7). */
 int yh_instruct(yh_reg *reg, const yh_op *op);
 /* Host-side gate matrix, as Gate::matrix (gate.rs:63-68, 284-393); out holds

@@ -130,3 +130,16 @@ def test_no_device_here_fails_loudly():
     h = ctypes.c_void_p()
     st = A.lib().yh_reg_create(4, A.C128, ctypes.byref(h))
     assert st == 14
+
+
+def test_tile_plan_covers_every_gate():
+    """Host-only planner: every gate of the 28-qubit depth-20 brickwork lands in a
+    tile pass, with far fewer passes than gates."""
+    from fixture_circuits import brickwork
+    els = brickwork(28, 20, 20)
+    ol = lower(els)
+    out = np.zeros(4, dtype=np.int64)
+    A.check(A.lib().yh_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
+    passes, groups, tiled, single = [int(x) for x in out]
+    assert tiled == len(els) and single == 0
+    assert passes <= 60 and groups >= passes

@@ -249,3 +249,51 @@ def test_qft12_config(gpu):
     assert abs(gpu.expect_arrayreg(reg, zz) - O.expect(n, want, zz)) < 1e-12
     u = np.random.default_rng(1024).random(1024)
     assert np.array_equal(gpu.sample(reg, None, u), O.sample(n, reg.state_vec(), None, u))
+
+
+# ---------------- tile passes (yh_apply_fused) ----------------
+@pytest.mark.parametrize("n,seed", [(12, 0), (13, 1), (15, 2), (16, 3)])
+def test_fused_random_circuits_vs_oracle(gpu, n, seed):
+    """All dispatch classes, including gates too wide for a tile (run on their own)."""
+    els = random_circuit(n, 250, 700 + seed, max_ctrls=3, custom_targets=(1, 2, 3, 4, 5, 6))
+    psi = seeded_state(n, seed)
+    print(f"[case] fused random n={n} seed={seed}", flush=True)
+    got = run_gpu(gpu, n, psi, els, fused=4)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+@pytest.mark.parametrize("n,depth", [(14, 6), (20, 4)])
+def test_fused_brickwork_vs_oracle(gpu, n, depth):
+    els = brickwork(n, depth, 40 + n)
+    psi = seeded_state(n, n)
+    got = run_gpu(gpu, n, psi, els, fused=4)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+def test_fused_qft_vs_oracle(gpu):
+    n = 16
+    psi = seeded_state(n, 16)
+    els = qft_with_swaps(n)
+    got = run_gpu(gpu, n, psi, els, fused=4)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+def test_fused_complex64_error(gpu):
+    n = 14
+    els = brickwork(n, 6, 3)
+    psi = seeded_state(n, 3)
+    got = run_gpu(gpu, n, psi, els, "c64", fused=4)
+    err = rel_err(got, O.apply(n, psi, lower(els)))
+    print(f"complex64 tiled relative error n={n}: {err:.3e}")
+    assert err < 2e-5
+
+
+def test_fused_matches_gate_by_gate_26q(gpu):
+    """Full-size cross-check between the two GPU paths (each pinned to the oracle
+    at smaller sizes): depth-4 brickwork on 26 qubits."""
+    n = 26
+    els = brickwork(n, 4, 8)
+    psi = seeded_state(n, 8)
+    a = run_gpu(gpu, n, psi, els, fused=0)
+    b = run_gpu(gpu, n, psi, els, fused=4)
+    assert rel_err(b, a) < TOL

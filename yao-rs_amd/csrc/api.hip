@@ -569,6 +569,25 @@ int yh_apply(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops) {
   return apply_gates<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
 }
 
+int yh_apply_fused(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops, int max_fuse_qubits) {
+  YH_GUARD(reg, YH_ERR_INVALID_ARG, "null register");
+  YH_GUARD(max_fuse_qubits >= 0 && max_fuse_qubits <= 5, YH_ERR_INVALID_ARG, "max_fuse_qubits must be in [0, 5]");
+  YH_GUARD(circuit_nbits == reg->nbits, YH_ERR_NBITS_MISMATCH, "Register and circuit qubit count mismatch");
+  YH_TRY(yh_validate(circuit_nbits, ops, nops));
+  if (reg->nbits < 12) {  // smaller than one tile: the per-gate path
+    if (reg->dtype == YH_C128) return apply_gates<double>(reg->w, reg->state.p, reg->nbits, ops, nops);
+    return apply_gates<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
+  }
+  if (reg->dtype == YH_C128) return apply_tiled<double>(reg->w, reg->state.p, reg->nbits, ops, nops);
+  return apply_tiled<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
+}
+
+int yh_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]) {
+  YH_GUARD(out, YH_ERR_INVALID_ARG, "null argument");
+  YH_TRY(yh_validate(nbits, ops, nops));
+  return plan_stats(nbits, ops, nops, out);
+}
+
 int yh_instruct(yh_reg* reg, const yh_op* op) {
   YH_GUARD(reg && op, YH_ERR_INVALID_ARG, "null argument");
   YH_GUARD(op->elem == YH_ELEM_GATE, YH_ERR_INVALID_ARG, "yh_instruct takes a gate element");

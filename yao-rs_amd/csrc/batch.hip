@@ -26,8 +26,4 @@ int yh_vqe_plain_energies(int ninst, const int32_t* nq, const int32_t* depth, co
   return fail(YH_ERR_UNSUPPORTED, "batched VQE is not built yet");
 }
 
-int yh_apply_fused(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops, int max_fuse_qubits) {
-  return fail(YH_ERR_UNSUPPORTED, "fused application is not built yet");
-}
-
 }  // extern "C"

@@ -39,6 +39,7 @@ struct Workspace {
   DevBuf probs;      // probabilities and their CDF
   DevBuf shots;      // caller-supplied uniforms
   DevBuf shots_out;  // sampled outcomes
+  DevBuf tiles;      // tile-pass groups and micro gates
   PinBuf pin;
   // launch timing (measurement hooks): events around each launch while enabled
   struct TimedLaunch {
@@ -68,5 +69,8 @@ int ensure_init();
 
 template <class R>
 int apply_gates(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
+template <class R>
+int apply_tiled(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
+int plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]);
 
 }  // namespace yh

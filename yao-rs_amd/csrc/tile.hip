@@ -1,0 +1,209 @@
+// tile.hip — multi-gate tile passes: one HBM read and write of the state applies
+// every gate that acts inside a 12-qubit tile.
+//
+// A workgroup loads its 2^12-amplitude tile into LDS (the 5 lowest memory bits plus
+// 7 high bits chosen per pass, so each load is a run of 512 B), then walks the
+// pass's gate groups: for each group of <= 4 tile qubits every thread pulls one
+// 16-amplitude sub-cube into registers, applies the group's gates one by one
+// (2x2 / 4x4 / diagonal, with controls local to the group), and writes it back.
+// The arithmetic of each gate is the per-gate kernels' arithmetic (yao-rs's
+// u1rows / unrows, src/instruct_qubit.rs:23-36, 189-206); only the HBM traffic
+// is shared between gates.
+#include "kernels.h"
+#include "tile.h"
+
+namespace yh {
+
+namespace {
+
+template <class R>
+__device__ __forceinline__ typename Cplx<R>::V ldm(const double (&m)[2]) {
+  return cmk<typename Cplx<R>::V>(R(m[0]), R(m[1]));
+}
+
+__device__ __forceinline__ int lpad(int p) { return p + (p >> 4); }
+
+template <class R, int T>
+__device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
+  using V = typename Cplx<R>::V;
+  const V m0 = ldm<R>(g.m[0]), m1 = ldm<R>(g.m[1]), m2 = ldm<R>(g.m[2]), m3 = ldm<R>(g.m[3]);
+  const int cm = g.cmask, cv = g.cval;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (i & (1 << T)) continue;
+    const int j = i | (1 << T);
+    if ((i & cm) == cv) {
+      const V w = a[i], v = a[j];
+      a[i] = cmul2(m0, w, m1, v);
+      a[j] = cmul2(m2, w, m3, v);
+    }
+  }
+}
+
+template <class R, int T>
+__device__ __forceinline__ void mg_diag1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
+  using V = typename Cplx<R>::V;
+  const V d0 = ldm<R>(g.m[0]), d1 = ldm<R>(g.m[1]);
+  const int cm = g.cmask, cv = g.cval;
+#pragma unroll
+  for (int i = 0; i < 16; ++i)
+    if ((i & cm) == cv) a[i] = cmul(a[i], (i & (1 << T)) ? d1 : d0);
+}
+
+template <class R, int T0, int T1>
+__device__ __forceinline__ void mg_dense2(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
+  using V = typename Cplx<R>::V;
+  V m[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e) m[e] = ldm<R>(g.m[e]);
+  const int cm = g.cmask, cv = g.cval;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (i & ((1 << T0) | (1 << T1))) continue;
+    if ((i & cm) != cv) continue;
+    const int idx[4] = {i, i | (1 << T0), i | (1 << T1), i | (1 << T0) | (1 << T1)};
+    V in[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) in[c] = a[idx[c]];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      V acc = cmul(m[r * 4], in[0]);
+#pragma unroll
+      for (int c = 1; c < 4; ++c) acc = cadd(acc, cmul(m[r * 4 + c], in[c]));
+      a[idx[r]] = acc;
+    }
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
+  switch (g.kind) {
+    case MG_DENSE1:
+      switch (g.t0) {
+        case 0: mg_dense1<R, 0>(a, g); break;
+        case 1: mg_dense1<R, 1>(a, g); break;
+        case 2: mg_dense1<R, 2>(a, g); break;
+        default: mg_dense1<R, 3>(a, g); break;
+      }
+      break;
+    case MG_DIAG1:
+      switch (g.t0) {
+        case 0: mg_diag1<R, 0>(a, g); break;
+        case 1: mg_diag1<R, 1>(a, g); break;
+        case 2: mg_diag1<R, 2>(a, g); break;
+        default: mg_diag1<R, 3>(a, g); break;
+      }
+      break;
+    default:
+      switch (g.t0 * 4 + g.t1) {
+        case 1: mg_dense2<R, 0, 1>(a, g); break;
+        case 2: mg_dense2<R, 0, 2>(a, g); break;
+        case 3: mg_dense2<R, 0, 3>(a, g); break;
+        case 4: mg_dense2<R, 1, 0>(a, g); break;
+        case 6: mg_dense2<R, 1, 2>(a, g); break;
+        case 7: mg_dense2<R, 1, 3>(a, g); break;
+        case 8: mg_dense2<R, 2, 0>(a, g); break;
+        case 9: mg_dense2<R, 2, 1>(a, g); break;
+        case 11: mg_dense2<R, 2, 3>(a, g); break;
+        case 12: mg_dense2<R, 3, 0>(a, g); break;
+        case 13: mg_dense2<R, 3, 1>(a, g); break;
+        default: mg_dense2<R, 3, 2>(a, g); break;
+      }
+      break;
+  }
+}
+
+template <class R>
+__global__ __launch_bounds__(kTileThreads) void k_tile(typename Cplx<R>::V* __restrict__ s, TilePass pass,
+                                                       LockedBits tile_lk, const TileGroup* __restrict__ groups,
+                                                       const MicroGate* __restrict__ mg) {
+  using V = typename Cplx<R>::V;
+  __shared__ V lds[kTileN + kTileN / 16];
+  __shared__ uint64_t hoff[1 << kTileHigh];
+  const int tid = threadIdx.x;
+  const uint64_t base = insert_zeros(uint64_t(blockIdx.x), tile_lk.pos, tile_lk.npos);
+  if (tid < (1 << kTileHigh)) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < kTileHigh; ++k)
+      if ((tid >> k) & 1) o |= uint64_t(1) << pass.hbits[k];
+    hoff[tid] = o;
+  }
+  __syncthreads();
+  V v[kAmpsPerThread];
+#pragma unroll
+  for (int k = 0; k < kAmpsPerThread; ++k) {
+    const int p = tid + k * kTileThreads;
+    v[k] = s[base + (p & ((1 << kTileLow) - 1)) + hoff[p >> kTileLow]];
+  }
+#pragma unroll
+  for (int k = 0; k < kAmpsPerThread; ++k) lds[lpad(tid + k * kTileThreads)] = v[k];
+  __syncthreads();
+
+  for (int gi = 0; gi < pass.ngroups; ++gi) {
+    const TileGroup& G = groups[pass.first_group + gi];
+    int qs[kGroupBits];
+#pragma unroll
+    for (int k = 0; k < kGroupBits; ++k) qs[k] = G.q[k];
+    // sub-cube base: insert zeros at the group's tile qubits (ascending) into tid
+    int sorted[kGroupBits] = {qs[0], qs[1], qs[2], qs[3]};
+#pragma unroll
+    for (int x = 0; x < kGroupBits; ++x)
+#pragma unroll
+      for (int y = 0; y < kGroupBits - 1 - x; ++y)
+        if (sorted[y] > sorted[y + 1]) {
+          int t = sorted[y];
+          sorted[y] = sorted[y + 1];
+          sorted[y + 1] = t;
+        }
+    int b = tid;
+#pragma unroll
+    for (int k = 0; k < kGroupBits; ++k) b = ((b >> sorted[k]) << (sorted[k] + 1)) | (b & ((1 << sorted[k]) - 1));
+    V a[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int p = b | ((i & 1) << qs[0]) | (((i >> 1) & 1) << qs[1]) | (((i >> 2) & 1) << qs[2]) |
+                    (((i >> 3) & 1) << qs[3]);
+      a[i] = lds[lpad(p)];
+    }
+    for (int k = 0; k < G.count; ++k) run_micro_gate<R>(a, mg[G.first + k]);
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int p = b | ((i & 1) << qs[0]) | (((i >> 1) & 1) << qs[1]) | (((i >> 2) & 1) << qs[2]) |
+                    (((i >> 3) & 1) << qs[3]);
+      lds[lpad(p)] = a[i];
+    }
+    __syncthreads();
+  }
+
+#pragma unroll
+  for (int k = 0; k < kAmpsPerThread; ++k) v[k] = lds[lpad(tid + k * kTileThreads)];
+#pragma unroll
+  for (int k = 0; k < kAmpsPerThread; ++k) {
+    const int p = tid + k * kTileThreads;
+    s[base + (p & ((1 << kTileLow) - 1)) + hoff[p >> kTileLow]] = v[k];
+  }
+}
+
+}  // namespace
+
+template <class R>
+int launch_tile(typename Cplx<R>::V* s, int nbits, const TilePass& pass, const TileGroup* groups,
+                const MicroGate* mg, hipStream_t st) {
+  LockedBits lk{};
+  int pos[kTileBits];
+  for (int k = 0; k < kTileLow; ++k) pos[k] = k;
+  for (int k = 0; k < kTileHigh; ++k) pos[kTileLow + k] = pass.hbits[k];
+  for (int a = 1; a < kTileBits; ++a)
+    for (int b = a; b > 0 && pos[b - 1] > pos[b]; --b) std::swap(pos[b - 1], pos[b]);
+  for (int k = 0; k < kTileBits; ++k) lk.pos[k] = uint8_t(pos[k]);
+  lk.npos = kTileBits;
+  const unsigned ntiles = unsigned(uint64_t(1) << (nbits - kTileBits));
+  hipLaunchKernelGGL(k_tile<R>, dim3(ntiles), dim3(kTileThreads), 0, st, s, pass, lk, groups, mg);
+  return YH_OK;
+}
+
+template int launch_tile<double>(double2*, int, const TilePass&, const TileGroup*, const MicroGate*, hipStream_t);
+template int launch_tile<float>(float2*, int, const TilePass&, const TileGroup*, const MicroGate*, hipStream_t);
+
+}  // namespace yh
