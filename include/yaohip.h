@@ -209,7 +209,8 @@ enum yh_kclass {
   YH_K_FUSED = 6,   /* fused 2^k block (MFMA)            */
   YH_K_TILE = 7,    /* multi-gate LDS tile pass          */
   YH_K_DM = 8,      /* density-matrix block pass         */
-  YH_K_NCLASS = 9
+  YH_K_VQE = 9,     /* batched VQE persistent launch      */
+  YH_K_NCLASS = 10
 };
 int yh_reg_timing_enable(yh_reg *reg, int on);
 int yh_reg_timing_read(const yh_reg *reg, int kclass, uint64_t *launches, double *total_ms);

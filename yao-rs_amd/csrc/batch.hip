@@ -1,29 +1,600 @@
-// batch.hip — batched VQE energies (many ansatz instances per launch).
+// batch.hip — batched VQE energies: thousands of hardware-efficient ansatz
+// instances of uneven qubit counts in one persistent launch.
+//
+// Ansatz (per layer): Ry then Rz on every qubit, then CZ(i, i+1) for i < n-1
+// (CZ = control([i], [i+1], Z), yao-rs easybuild.rs:516), from |0...0>.
+// Energy: H = -J sum_i Z_i Z_{i+1} - h sum_i X_i (open chain), FP64 reductions.
+//
+// Each workgroup pulls instances from an atomic queue (largest first). The
+// per-qubit Rz*Ry product of a layer is composed in FP64 on the host, so a layer
+// is n single-qubit 2x2 updates followed by the CZ chain, which is the exact sign
+// (-1)^popcount(x & x>>1) and is applied in the same register pass as the last
+// rotation group. A state of at most 2^TB amplitudes lives in LDS for the whole
+// instance; a larger one streams through the workgroup's HBM slot in tiles of
+// 2^TB amplitudes: one pass holds bits 0..TB-1, the others bits 0..4 plus a chunk
+// of the high bits, so a layer costs 1 + ceil((n - TB) / (TB - 5)) passes.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "host.h"
 #include "kernels.h"
 #include "runtime.h"
+
+namespace yh {
+
+struct VqeInst {
+  int32_t n, depth;
+  int64_t rot_off;  // complex entries: 4 per (layer, bit)
+  int32_t out;      // energy slot
+  int32_t pad;
+};
+
+namespace {
+
+constexpr int kVqeLow = 5;
+
+template <int TB>
+__device__ __forceinline__ int vpad(int p) {
+  return p + (p >> 4);
+}
+
+// streaming (non-temporal) access to the per-workgroup HBM slot
+template <class V>
+__device__ __forceinline__ V nt_load(const V* p) {
+  using R = decltype(V::x);
+  const R* q = reinterpret_cast<const R*>(p);
+  return cmk<V>(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+}
+template <class V>
+__device__ __forceinline__ void nt_store(V v, V* p) {
+  using R = decltype(V::x);
+  R* q = reinterpret_cast<R*>(p);
+  __builtin_nontemporal_store(R(v.x), q);
+  __builtin_nontemporal_store(R(v.y), q + 1);
+}
+
+__device__ __forceinline__ double cz_sign(uint64_t x, int n) {
+  const uint64_t m = (n >= 2) ? ((uint64_t(1) << (n - 1)) - 1) : 0;
+  return (__popcll(x & (x >> 1) & m) & 1) ? -1.0 : 1.0;
+}
+__device__ __forceinline__ double zz_of(uint64_t x, int n) {
+  const uint64_t m = (n >= 2) ? ((uint64_t(1) << (n - 1)) - 1) : 0;
+  return double(n - 1) - 2.0 * double(__popcll((x ^ (x >> 1)) & m));
+}
+
+template <class R>
+__device__ __forceinline__ void rot_local(typename Cplx<R>::V (&a)[16], int k, const typename Cplx<R>::V* u) {
+  using V = typename Cplx<R>::V;
+  const V m0 = u[0], m1 = u[1], m2 = u[2], m3 = u[3];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (((i >> k) & 1) == 0) {
+      const int j = i | (1 << k);
+      const V w = a[i], v = a[j];
+      a[i] = cmul2(m0, w, m1, v);
+      a[j] = cmul2(m2, w, m3, v);
+    }
+  }
+}
+
+template <class R>
+__device__ __forceinline__ void rot_group(typename Cplx<R>::V (&a)[16], int gb, const typename Cplx<R>::V* u4) {
+  // u4: the 2x2 matrices of the group's bits, 4 entries each
+  if (gb > 0) rot_local<R>(a, 0, u4);
+  if (gb > 1) rot_local<R>(a, 1, u4 + 4);
+  if (gb > 2) rot_local<R>(a, 2, u4 + 8);
+  if (gb > 3) rot_local<R>(a, 3, u4 + 12);
+}
+
+__device__ __forceinline__ double bred(double v, double* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < int(blockDim.x >> 6); ++w) r += sh[w];
+  return r;
+}
+
+// memory bit of tile position p for a pass whose high positions map to hb[]
+__device__ __forceinline__ uint64_t tile_addr(int p, const uint8_t* hb, int nh) {
+  uint64_t a = uint64_t(p & ((1 << kVqeLow) - 1));
+  for (int k = 0; k < nh; ++k) a |= uint64_t((p >> (kVqeLow + k)) & 1) << hb[k];
+  return a;
+}
+
+template <class R, int TB>
+__global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, int ninst, int* __restrict__ counter,
+                                             const typename Cplx<R>::V* __restrict__ rot,
+                                             typename Cplx<R>::V* __restrict__ scratch, uint64_t slot_amps,
+                                             double* __restrict__ energies, double J, double h) {
+  using V = typename Cplx<R>::V;
+  constexpr int TN = 1 << TB;
+  __shared__ V lds[TN + TN / 16];
+  __shared__ double red[16];
+  __shared__ int cur;
+  const int tid = threadIdx.x, nt = blockDim.x;
+  V* slot = scratch + uint64_t(blockIdx.x) * slot_amps;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) cur = atomicAdd(counter, 1);
+    __syncthreads();
+    const int ii = cur;
+    if (ii >= ninst) break;  // every thread reads the same value: all waves leave together
+    const VqeInst in = insts[ii];
+    const int n = in.n, depth = in.depth;
+    const V* rt = rot + in.rot_off;
+    double zz_acc = 0.0, x_acc = 0.0;
+    if (n <= TB) {
+      // ---------------- whole state in LDS ----------------
+      const int N = 1 << n;
+      for (int x = tid; x < N; x += nt) lds[vpad<TB>(x)] = cmk<V>(R(x == 0 ? 1 : 0), R(0));
+      __syncthreads();
+      for (int layer = 0; layer < depth; ++layer) {
+        const V* ul = rt + int64_t(layer) * n * 4;
+        for (int g0 = 0; g0 < n; g0 += 4) {
+          const int gb = min(4, n - g0);
+          const bool last = g0 + 4 >= n;
+          const int nsub = N >> gb;
+          for (int sc = tid; sc < nsub; sc += nt) {
+            const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+            V a[16];
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+              if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+            rot_group<R>(a, gb, ul + g0 * 4);
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+              if (i < (1 << gb)) {
+                const int x = base + (i << g0);
+                V v = a[i];
+                if (last && cz_sign(uint64_t(x), n) < 0) {
+                  v.x = -v.x;
+                  v.y = -v.y;
+                }
+                lds[vpad<TB>(x)] = v;
+              }
+          }
+          __syncthreads();
+        }
+      }
+      for (int x = tid; x < N; x += nt) {
+        const V p = lds[vpad<TB>(x)];
+        const double pr = double(p.x), pi = double(p.y);
+        zz_acc += (pr * pr + pi * pi) * zz_of(uint64_t(x), n);
+        for (int b = 0; b < n; ++b) {
+          const V q = lds[vpad<TB>(x ^ (1 << b))];
+          x_acc += pr * double(q.x) + pi * double(q.y);
+        }
+      }
+    } else {
+      // ---------------- state streams through the HBM slot ----------------
+      const int nchunk = (n - TB + (TB - kVqeLow) - 1) / (TB - kVqeLow);
+      const int npass = 1 + nchunk;
+      const uint64_t ntile = uint64_t(1) << (n - TB);
+      __shared__ uint8_t hb[TB];
+      __shared__ uint8_t lkpos[TB];
+      for (int layer = 0; layer <= depth; ++layer) {  // layer == depth: energy passes (read only)
+        const V* ul = rt + int64_t(layer < depth ? layer : 0) * n * 4;
+        for (int ps = 0; ps < npass; ++ps) {
+          // tile bits: pass 0 -> 0..TB-1; pass k -> 0..4 plus chunk k of [TB, n)
+          int nh, lo = 0;
+          if (tid == 0) {
+            if (ps == 0) {
+              for (int k = 0; k < TB - kVqeLow; ++k) hb[k] = uint8_t(kVqeLow + k);
+            } else {
+              lo = TB + (ps - 1) * (TB - kVqeLow);
+              for (int k = 0; k < TB - kVqeLow; ++k) hb[k] = uint8_t(min(lo + k, n - 1));
+            }
+          }
+          __syncthreads();
+          if (ps == 0) nh = TB - kVqeLow;
+          else {
+            lo = TB + (ps - 1) * (TB - kVqeLow);
+            nh = min(TB - kVqeLow, n - lo);
+          }
+          const int tbits = kVqeLow + nh;  // tile qubits in this pass
+          // sorted locked bits of the tile (for the tile base)
+          if (tid == 0) {
+            int c = 0;
+            for (int b = 0; b < kVqeLow; ++b) lkpos[c++] = uint8_t(b);
+            for (int k = 0; k < nh; ++k) lkpos[c++] = hb[k];
+            for (int a = 1; a < c; ++a)
+              for (int b2 = a; b2 > 0 && lkpos[b2 - 1] > lkpos[b2]; --b2) {
+                uint8_t t = lkpos[b2 - 1];
+                lkpos[b2 - 1] = lkpos[b2];
+                lkpos[b2] = t;
+              }
+          }
+          __syncthreads();
+          const int TNp = 1 << tbits;
+          const uint64_t ntp = uint64_t(1) << (n - tbits);
+          // rotation bits handled by this pass: pass 0 all its bits, others only the chunk
+          const int rb_first = ps == 0 ? 0 : kVqeLow;
+          const bool last_pass = ps == npass - 1;
+          for (uint64_t t = 0; t < ntp; ++t) {
+            uint64_t tb = t;
+            for (int k = 0; k < tbits; ++k) {
+              const int pp = lkpos[k];
+              tb = ((tb >> pp) << (pp + 1)) | (tb & ((uint64_t(1) << pp) - 1));
+            }
+            for (int p = tid; p < TNp; p += nt) {
+              const uint64_t x = tb + tile_addr(p, hb, nh);
+              V v;
+              // |0...0> is generated, not read, until the first pass has written the slot
+              if (depth == 0 || (layer == 0 && ps == 0)) v = cmk<V>(R(x == 0 ? 1 : 0), R(0));
+              else v = nt_load(&slot[x]);
+              lds[vpad<TB>(p)] = v;
+            }
+            __syncthreads();
+            if (layer < depth) {
+              for (int g0 = rb_first; g0 < tbits; g0 += 4) {
+                const int gb = min(4, tbits - g0);
+                const bool lastg = g0 + 4 >= tbits;
+                const int nsub = TNp >> gb;
+                // matrices of the group's memory bits
+                for (int sc = tid; sc < nsub; sc += nt) {
+                  const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+                  V a[16], u[16];
+#pragma unroll
+                  for (int k = 0; k < 4; ++k) {
+                    if (k < gb) {
+                      const int pos = g0 + k;
+                      const int mb = pos < kVqeLow ? pos : hb[pos - kVqeLow];
+#pragma unroll
+                      for (int e = 0; e < 4; ++e) u[4 * k + e] = ul[mb * 4 + e];
+                    }
+                  }
+#pragma unroll
+                  for (int i = 0; i < 16; ++i)
+                    if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+                  rot_group<R>(a, gb, u);
+#pragma unroll
+                  for (int i = 0; i < 16; ++i)
+                    if (i < (1 << gb)) {
+                      const int p = base + (i << g0);
+                      V v = a[i];
+                      if (last_pass && lastg && cz_sign(tb + tile_addr(p, hb, nh), n) < 0) {
+                        v.x = -v.x;
+                        v.y = -v.y;
+                      }
+                      lds[vpad<TB>(p)] = v;
+                    }
+                }
+                __syncthreads();
+              }
+              for (int p = tid; p < TNp; p += nt) {
+                const uint64_t x = tb + tile_addr(p, hb, nh);
+                nt_store(lds[vpad<TB>(p)], &slot[x]);
+              }
+            } else {
+              // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
+              for (int p = tid; p < TNp; p += nt) {
+                const V v = lds[vpad<TB>(p)];
+                const double pr = double(v.x), pi = double(v.y);
+                if (ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(tb + tile_addr(p, hb, nh), n);
+                for (int pos = rb_first; pos < tbits; ++pos) {
+                  const V q = lds[vpad<TB>(p ^ (1 << pos))];
+                  x_acc += pr * double(q.x) + pi * double(q.y);
+                }
+              }
+            }
+            __syncthreads();
+          }
+          __threadfence();
+          __syncthreads();
+        }
+      }
+    }
+    const double zz = bred(zz_acc, red);
+    const double xs = bred(x_acc, red);
+    if (tid == 0) energies[in.out] = -J * zz - h * xs;
+  }
+}
+
+__global__ void k_reset_counter(int* c) {
+  if (threadIdx.x == 0) *c = 0;
+}
+
+}  // namespace
+
+// Composed per-qubit layer matrix U = Rz(tz) * Ry(ty) (Ry first), FP64.
+static void rzry(double ty, double tz, cd u[4]) {
+  Mat ry, rz;
+  yh_op o{};
+  o.elem = YH_ELEM_GATE;
+  o.kind = YH_GATE_RY;
+  o.params[0] = ty;
+  gate_matrix(o, ry);
+  o.kind = YH_GATE_RZ;
+  o.params[0] = tz;
+  gate_matrix(o, rz);
+  for (int r = 0; r < 2; ++r)
+    for (int c = 0; c < 2; ++c) u[r * 2 + c] = rz[r * 2 + 0] * ry[0 * 2 + c] + rz[r * 2 + 1] * ry[1 * 2 + c];
+}
+
+struct VqeConfig {
+  int tile_bits = 12;
+  int blocks_per_cu = 2;
+  int threads = 256;
+};
+
+}  // namespace yh
 
 using namespace yh;
 
 struct yh_vqe_batch {
   int max_instances = 0, max_qubits = 0, dtype = YH_C64;
+  int cus = 256;
+  Workspace w;
+  DevBuf insts, rot, scratch, energies, counter;
+  VqeConfig cfg;
+  bool tuned = false;
+  hipGraphExec_t graph = nullptr;
+  int graph_ninst = -1;
+  double graph_J = 0, graph_h = 0;
+  VqeConfig graph_cfg;
+  ~yh_vqe_batch() {
+    if (graph) (void)hipGraphExecDestroy(graph);
+  }
 };
+
+namespace yh {
+
+template <class R>
+static int vqe_launch(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
+  using V = typename Cplx<R>::V;
+  const int grid = b->cus * cfg.blocks_per_cu;
+  const uint64_t slot = uint64_t(1) << b->max_qubits;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, static_cast<int*>(b->counter.p));
+  if (cfg.tile_bits == 12)
+    hipLaunchKernelGGL((k_vqe<R, 12>), dim3(grid), dim3(cfg.threads), 0, b->w.stream,
+                       static_cast<const VqeInst*>(b->insts.p), ninst, static_cast<int*>(b->counter.p),
+                       static_cast<const V*>(b->rot.p), static_cast<V*>(b->scratch.p), slot,
+                       static_cast<double*>(b->energies.p), J, h);
+  else
+    hipLaunchKernelGGL((k_vqe<R, 13>), dim3(grid), dim3(cfg.threads), 0, b->w.stream,
+                       static_cast<const VqeInst*>(b->insts.p), ninst, static_cast<int*>(b->counter.p),
+                       static_cast<const V*>(b->rot.p), static_cast<V*>(b->scratch.p), slot,
+                       static_cast<double*>(b->energies.p), J, h);
+  return YH_OK;
+}
+
+static int vqe_run(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
+  if (b->dtype == YH_C128) return vqe_launch<double>(b, ninst, J, h, cfg);
+  return vqe_launch<float>(b, ninst, J, h, cfg);
+}
+
+static std::vector<VqeConfig> vqe_candidates(int dtype) {
+  std::vector<VqeConfig> c;
+  for (int tb : {12, 13}) {
+    if (dtype == YH_C128 && tb == 13) continue;  // 128 KB of LDS: one workgroup per CU
+    for (int bpc : {1, 2, 4})
+      for (int th : {256}) {
+        VqeConfig v;
+        v.tile_bits = tb;
+        v.blocks_per_cu = bpc;
+        v.threads = th;
+        c.push_back(v);
+      }
+  }
+  return c;
+}
+
+}  // namespace yh
 
 extern "C" {
 
 int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_batch** out) {
-  return fail(YH_ERR_UNSUPPORTED, "batched VQE is not built yet");
+  if (!out) return fail(YH_ERR_INVALID_ARG, "null output pointer");
+  if (max_instances < 1 || max_qubits < 1 || max_qubits > 24)
+    return fail(YH_ERR_INVALID_ARG, "max_instances >= 1 and max_qubits in [1, 24]");
+  if (dtype != YH_C64 && dtype != YH_C128) return fail(YH_ERR_DTYPE, "unknown dtype");
+  YH_TRY(ensure_init());
+  yh_vqe_batch* b = new yh_vqe_batch();
+  b->max_instances = max_instances;
+  b->max_qubits = max_qubits;
+  b->dtype = dtype;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, 0) == hipSuccess) b->cus = prop.multiProcessorCount;
+  b->cfg.tile_bits = dtype == YH_C64 ? 13 : 12;
+  const size_t amp = dtype == YH_C128 ? 16 : 8;
+  const int max_grid = b->cus * 4;
+  int st = b->w.init();
+  if (st == YH_OK) st = b->insts.ensure(sizeof(VqeInst) * max_instances);
+  if (st == YH_OK) st = b->energies.ensure(sizeof(double) * max_instances);
+  if (st == YH_OK) st = b->counter.ensure(256);
+  if (st == YH_OK) st = b->scratch.ensure(amp * (size_t(1) << max_qubits) * max_grid);
+  if (st != YH_OK) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return YH_OK;
 }
+
 int yh_vqe_batch_destroy(yh_vqe_batch* b) {
   delete b;
   return YH_OK;
 }
-int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth,
-                          const int64_t* offset, const double* angles, double J, double h, double* energies) {
-  return fail(YH_ERR_UNSUPPORTED, "batched VQE is not built yet");
+
+// Uploads the instance table and the FP64-composed layer matrices.
+static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
+                      const double* angles) {
+  std::vector<VqeInst> ins(ninst);
+  std::vector<cd> rot;
+  for (int i = 0; i < ninst; ++i) {
+    const int n = nq[i], d = depth[i];
+    if (n < 1 || n > b->max_qubits || d < 0) return fail(YH_ERR_INVALID_ARG, "instance qubit count or depth out of range");
+    ins[i].n = n;
+    ins[i].depth = d;
+    ins[i].out = i;
+    ins[i].rot_off = int64_t(rot.size());
+    for (int l = 0; l < d; ++l)
+      for (int bit = 0; bit < n; ++bit) {
+        const int loc = n - 1 - bit;
+        const double* a = angles + offset[i] + (int64_t(l) * n + loc) * 2;
+        cd u[4];
+        rzry(a[0], a[1], u);
+        rot.insert(rot.end(), u, u + 4);
+      }
+    if (d == 0) rot.insert(rot.end(), 4, cd{0, 0});  // keep rot_off valid
+  }
+  // largest instances first (longest-processing-time order)
+  std::vector<int> order(ninst);
+  std::iota(order.begin(), order.end(), 0);
+  auto cost = [&](int i) { return double(int64_t(1) << ins[i].n) * double(ins[i].depth + 1) * ins[i].n; };
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost(a) > cost(c); });
+  std::vector<VqeInst> sorted(ninst);
+  for (int k = 0; k < ninst; ++k) sorted[k] = ins[order[k]];
+  YH_TRY(b->w.upload(sorted.data(), sorted.size() * sizeof(VqeInst), b->insts));
+  if (b->dtype == YH_C128) {
+    YH_TRY(b->w.upload(rot.data(), rot.size() * sizeof(cd), b->rot));
+  } else {
+    std::vector<float> f(2 * rot.size());
+    for (size_t k = 0; k < rot.size(); ++k) {
+      f[2 * k] = float(rot[k].re);
+      f[2 * k + 1] = float(rot[k].im);
+    }
+    YH_TRY(b->w.upload(f.data(), f.size() * sizeof(float), b->rot));
+  }
+  return YH_OK;
 }
+
+int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
+                          const double* angles, double J, double h, double* energies) {
+  if (!b || !nq || !depth || !offset || !angles || !energies) return fail(YH_ERR_INVALID_ARG, "null argument");
+  if (ninst < 1 || ninst > b->max_instances) return fail(YH_ERR_INVALID_ARG, "instance count out of range");
+  YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles));
+  if (!b->tuned) {
+    // autotune: time every candidate launch shape on this batch, keep the fastest
+    double best = 1e30;
+    for (const VqeConfig& c : vqe_candidates(b->dtype)) {
+      YH_TRY(vqe_run(b, ninst, J, h, c));  // warm
+      YH_TRY(b->w.sync());
+      auto t0 = std::chrono::steady_clock::now();
+      YH_TRY(vqe_run(b, ninst, J, h, c));
+      YH_TRY(b->w.sync());
+      double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (dt < best) {
+        best = dt;
+        b->cfg = c;
+      }
+    }
+    b->tuned = true;
+  }
+  // replay a captured graph of (counter reset, persistent kernel) for this shape
+  if (!b->graph || b->graph_ninst != ninst || b->graph_cfg.tile_bits != b->cfg.tile_bits ||
+      b->graph_cfg.blocks_per_cu != b->cfg.blocks_per_cu) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    YH_HIP(hipStreamSynchronize(b->w.stream));
+    hipGraph_t g;
+    YH_HIP(hipStreamBeginCapture(b->w.stream, hipStreamCaptureModeThreadLocal));
+    int st = vqe_run(b, ninst, J, h, b->cfg);
+    hipError_t e = hipStreamEndCapture(b->w.stream, &g);
+    if (st != YH_OK) return st;
+    if (e != hipSuccess) return fail(YH_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
+    e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(YH_ERR_HIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
+    b->graph_ninst = ninst;
+    b->graph_cfg = b->cfg;
+    b->graph_J = J;
+    b->graph_h = h;
+  }
+  if (b->graph_J != J || b->graph_h != h) {
+    YH_TRY(vqe_run(b, ninst, J, h, b->cfg));
+  } else {
+    YH_TRY(b->w.time_begin(YH_K_VQE));
+    YH_HIP(hipGraphLaunch(b->graph, b->w.stream));
+    YH_TRY(b->w.time_end());
+  }
+  return b->w.fetch(energies, b->energies.p, sizeof(double) * ninst);
+}
+
+// The plain per-circuit GPU path: every instance through yh_apply (one launch per
+// gate) and yh_expect, as a caller of the drop-in API would evaluate it.
 int yh_vqe_plain_energies(int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
                           const double* angles, double J, double h, int dtype, double* energies) {
-  return fail(YH_ERR_UNSUPPORTED, "batched VQE is not built yet");
+  if (!nq || !depth || !offset || !angles || !energies) return fail(YH_ERR_INVALID_ARG, "null argument");
+  std::vector<yh_reg*> regs(40, nullptr);
+  int st = YH_OK;
+  for (int i = 0; i < ninst && st == YH_OK; ++i) {
+    const int n = nq[i], d = depth[i];
+    if (n < 1 || n > 30) {
+      st = fail(YH_ERR_INVALID_ARG, "instance qubit count out of range");
+      break;
+    }
+    if (!regs[n]) {
+      st = yh_reg_create(n, dtype, &regs[n]);
+      if (st != YH_OK) break;
+    }
+    yh_reg* r = regs[n];
+    std::vector<double> zero(size_t(2) << n, 0.0);
+    zero[0] = 1.0;
+    st = yh_reg_upload(r, zero.data(), size_t(1) << n);
+    if (st != YH_OK) break;
+    std::vector<yh_op> ops;
+    for (int l = 0; l < d; ++l) {
+      for (int q = 0; q < n; ++q) {
+        const double* a = angles + offset[i] + (int64_t(l) * n + q) * 2;
+        yh_op o{};
+        o.elem = YH_ELEM_GATE;
+        o.nt = 1;
+        o.targets[0] = q;
+        o.kind = YH_GATE_RY;
+        o.params[0] = a[0];
+        ops.push_back(o);
+        o.kind = YH_GATE_RZ;
+        o.params[0] = a[1];
+        ops.push_back(o);
+      }
+      for (int q = 0; q + 1 < n; ++q) {
+        yh_op o{};
+        o.elem = YH_ELEM_GATE;
+        o.kind = YH_GATE_Z;
+        o.nt = 1;
+        o.targets[0] = q + 1;
+        o.nc = o.nconf = 1;
+        o.ctrls[0] = q;
+        o.ctrl_vals[0] = 1;
+        ops.push_back(o);
+      }
+    }
+    st = yh_apply(r, n, ops.data(), ops.size());
+    if (st != YH_OK) break;
+    // TFIM terms
+    std::vector<double> coeffs;
+    std::vector<int32_t> tl, sites, opc;
+    for (int q = 0; q + 1 < n; ++q) {
+      coeffs.push_back(-J);
+      coeffs.push_back(0);
+      tl.push_back(2);
+      sites.push_back(q);
+      opc.push_back(YH_OP_Z);
+      sites.push_back(q + 1);
+      opc.push_back(YH_OP_Z);
+    }
+    for (int q = 0; q < n; ++q) {
+      coeffs.push_back(-h);
+      coeffs.push_back(0);
+      tl.push_back(1);
+      sites.push_back(q);
+      opc.push_back(YH_OP_X);
+    }
+    double e[2];
+    st = yh_expect(r, tl.size(), coeffs.data(), tl.data(), sites.data(), opc.data(), e);
+    energies[i] = e[0];
+  }
+  for (auto* r : regs)
+    if (r) yh_reg_destroy(r);
+  return st;
 }
 
 }  // extern "C"

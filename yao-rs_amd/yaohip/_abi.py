@@ -29,8 +29,8 @@ GATE_NAMES = ["X", "Y", "Z", "H", "S", "T", "SWAP", "Phase", "Rx", "Ry", "Rz", "
  CH_PHASE_DAMPING, CH_PHASE_AMPLITUDE_DAMPING, CH_THERMAL_RELAXATION, CH_COHERENT,
  CH_CUSTOM) = range(11)
 OP_I, OP_X, OP_Y, OP_Z, OP_P0, OP_P1, OP_PU, OP_PD = range(8)
-(K_1Q, K_DENSE, K_DIAG, K_X, K_SWAP, K_OOP, K_FUSED, K_TILE, K_DM) = range(9)
-KCLASS_NAMES = ["1q", "dense", "diag", "x", "swap", "oop", "fused", "tile", "dm"]
+(K_1Q, K_DENSE, K_DIAG, K_X, K_SWAP, K_OOP, K_FUSED, K_TILE, K_DM, K_VQE) = range(10)
+KCLASS_NAMES = ["1q", "dense", "diag", "x", "swap", "oop", "fused", "tile", "dm", "vqe"]
 MAX_TARGETS = 12
 MAX_CTRLS = 16
 
