@@ -77,6 +77,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-extra", action="store_true", help="only the 28-qubit complex128 brickwork")
     ap.add_argument("--nbits", type=int, default=NBITS)
     ap.add_argument("--depth", type=int, default=DEPTH)
     args = ap.parse_args()
@@ -158,11 +159,121 @@ def main():
                          "roofline": roofline_of(gbg_k),
                          "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in gbg_k.items()}},
     }
+    # complex64 variant of the same circuit; error against the FP64 result (the
+    # complex128 path is pinned to the oracle at 1e-12 by the parity tests)
+    if not args.skip_extra:
+        del reg
+        reg = yaohip.ArrayReg.zero_state(n)
+        yaohip.apply_inplace(circ, reg, 4)
+        ref = reg.state_vec()
+        reg64 = yaohip.ArrayReg.zero_state(n, "c64")
+        yaohip.apply_inplace(circ, reg64, 4)
+        t0 = time.perf_counter()
+        steps64 = max(1, args.steps)
+        for _ in range(steps64):
+            yaohip.apply_inplace(circ, reg64, 4)
+        dt64 = time.perf_counter() - t0
+        reg64b = yaohip.ArrayReg.zero_state(n, "c64")
+        yaohip.apply_inplace(circ, reg64b, 4)
+        got64 = reg64b.state_vec()
+        err64 = float(np.max(np.abs(got64 - ref)) / np.max(np.abs(ref)))
+        fid64 = float(abs(np.vdot(ref, got64)) ** 2)
+        out["complex64"] = {"value": round(ngates * steps64 / dt64, 2), "unit": "gate-apps/s",
+                            "ms_per_step": round(1e3 * dt64 / steps64, 3),
+                            "max_rel_err_vs_fp64": err64, "fidelity_vs_fp64": fid64,
+                            "dtype": "f32 amplitudes, f64 host-fused matrices"}
+        log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
+        del reg64, reg64b, got64, ref
+        out["vqe_batched"] = bench_vqe(log)
+        out["noisy_dm"] = bench_dm(log)
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
         del reg
         out["cpu_baseline"] = cpu_baseline(els, n)
     print(json.dumps(out), flush=True)
+
+
+def bench_vqe(log, count=4096, plain_sample=256):
+    """Batched VQE: 4096 ansatz instances, n ~ U{10..20}, depth ~ U{4..8}, angles
+    U[-pi, pi], TFIM J = h = 1. evals/s of the one-launch batch (complex64 and
+    complex128) against the plain per-circuit GPU path (yh_apply + yh_expect per
+    instance), timed on a sample of instances and scaled to the batch."""
+    from yaohip.vqe import VqeBatch, plain_energies, random_instances
+    inst = random_instances(count, seed=4)
+    res = {"instances": count}
+    energies = {}
+    for dt in ("c128", "c64"):
+        b = VqeBatch(count, 20, dt)
+        b.energies(inst)  # autotune + graph capture
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            e = b.energies(inst)
+        el = (time.perf_counter() - t0) / reps
+        energies[dt] = e
+        res[f"batched_{dt}_evals_per_s"] = round(count / el, 1)
+        res[f"batched_{dt}_ms"] = round(1e3 * el, 3)
+        log(f"vqe batched {dt}: {1e3 * el:.2f} ms for {count} instances")
+        del b
+    d = np.abs(energies["c64"] - energies["c128"])
+    res["c64_abs_dE_vs_fp64"] = {"max": float(d.max()), "median": float(np.median(d)),
+                                 "p99": float(np.percentile(d, 99)), "frac_within_1e-6": float(np.mean(d <= 1e-6))}
+    # the plain per-circuit path on a sample (same instance mix: every k-th instance)
+    idx = np.arange(0, count, count // plain_sample)[:plain_sample]
+    from yaohip.vqe import VqeInstances
+    sub = VqeInstances(inst.nq[idx], inst.depth[idx], [inst.instance_angles(i) for i in idx])
+    plain_energies(VqeInstances(sub.nq[:4], sub.depth[:4], [sub.instance_angles(i) for i in range(4)]))
+    t0 = time.perf_counter()
+    pe = plain_energies(sub, dtype="c128")
+    tp = time.perf_counter() - t0
+    res["plain_c128_evals_per_s"] = round(len(idx) / tp, 1)
+    res["plain_sample"] = f"{len(idx)} instances (every {count // plain_sample}th), {tp:.2f} s"
+    res["plain_vs_batched_c128_max_abs_dE"] = float(np.max(np.abs(pe - energies['c128'][idx])))
+    ok64 = res["c64_abs_dE_vs_fp64"]["max"] <= 1e-6
+    best = "c64" if ok64 else "c128"
+    res["reported_path"] = f"batched {best} (max |dE| vs FP64 {'<=' if ok64 else '>'} 1e-6 for c64)"
+    res["evals_per_s_at_dE_1e-6"] = res[f"batched_{best}_evals_per_s"]
+    res["speedup_vs_plain"] = round(res[f"batched_{best}_evals_per_s"] / res["plain_c128_evals_per_s"], 1)
+    log(f"vqe: {res}")
+    return res
+
+
+def bench_dm(log, n=14, depth=10):
+    """14-qubit noisy density matrix (2^28 complex128 entries), brickwork depth 10 with
+    Depolarizing(1e-3) then AmplitudeDamping(1e-3) after every gate on each touched
+    qubit; gate and its channels fused into one pass. Outputs <Z_i Z_i+1>, trace, purity."""
+    import yaohip
+    from yaohip import _abi as A
+    from yaohip.circuit import Circuit, OperatorPolynomial, OperatorString, Op
+    from fixture_circuits import noisy_brickwork
+    els = noisy_brickwork(n, depth, 3)
+    circ = Circuit.qubits(n, els)
+    ngates = sum(1 for e in els if hasattr(e, "gate"))
+    nch = len(els) - ngates
+    dm = yaohip.DensityMatrix.zero_state(n)
+    A.check(A.lib().yh_dm_timing_enable(dm._h, 1))
+    t0 = time.perf_counter()
+    dm.apply(circ, fused=True)
+    dt = time.perf_counter() - t0
+    c, ms = ctypes.c_uint64(), ctypes.c_double()
+    A.check(A.lib().yh_dm_timing_read(dm._h, A.K_DM, ctypes.byref(c), ctypes.byref(ms)))
+    tr = dm.trace()
+    pur = dm.purity()
+    zz = [gpu_zz(dm, i) for i in range(n - 1)]
+    bytes_pass = 2 * 16 * (1 << (2 * n))
+    avg = ms.value / max(c.value, 1)
+    res = {"nbits": n, "depth": depth, "gates": ngates, "channels": nch, "passes": int(c.value),
+           "elements_per_s": round(len(els) / dt, 1), "seconds": round(dt, 4),
+           "avg_pass_us": round(1e3 * avg, 2), "pass_gbps": round(bytes_pass / (avg * 1e-3) / 1e9, 1),
+           "trace": [tr.real, tr.imag], "purity": pur, "zz_lsb_sites": zz}
+    log(f"noisy dm: {res}")
+    return res
+
+
+def gpu_zz(dm, i):
+    import yaohip
+    from yaohip.circuit import OperatorPolynomial, OperatorString, Op
+    return yaohip.expect_dm(dm, OperatorPolynomial([1.0], [OperatorString([(i, Op.Z), (i + 1, Op.Z)])])).real
 
 
 if __name__ == "__main__":

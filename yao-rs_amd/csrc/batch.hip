@@ -108,11 +108,14 @@ __device__ __forceinline__ uint64_t tile_addr(int p, const uint8_t* hb, int nh) 
   return a;
 }
 
-template <class R, int TB>
-__global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, int ninst, int* __restrict__ counter,
-                                             const typename Cplx<R>::V* __restrict__ rot,
-                                             typename Cplx<R>::V* __restrict__ scratch, uint64_t slot_amps,
-                                             double* __restrict__ energies, double J, double h) {
+// BIG = false: instances [first, last) fit in LDS (n <= TB); BIG = true: they stream
+// through the workgroup's HBM slot. Two instantiations keep the register budgets apart.
+template <class R, int TB, bool BIG>
+__global__ __launch_bounds__(256, 2) void k_vqe(const VqeInst* __restrict__ insts, int first, int last,
+                                                int* __restrict__ counter,
+                                                const typename Cplx<R>::V* __restrict__ rot,
+                                                typename Cplx<R>::V* __restrict__ scratch, uint64_t slot_amps,
+                                                double* __restrict__ energies, double J, double h) {
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
   __shared__ V lds[TN + TN / 16];
@@ -122,15 +125,15 @@ __global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, 
   V* slot = scratch + uint64_t(blockIdx.x) * slot_amps;
   for (;;) {
     __syncthreads();
-    if (tid == 0) cur = atomicAdd(counter, 1);
+    if (tid == 0) cur = first + atomicAdd(counter, 1);
     __syncthreads();
     const int ii = cur;
-    if (ii >= ninst) break;  // every thread reads the same value: all waves leave together
+    if (ii >= last) break;  // every thread reads the same value: all waves leave together
     const VqeInst in = insts[ii];
     const int n = in.n, depth = in.depth;
     const V* rt = rot + in.rot_off;
     double zz_acc = 0.0, x_acc = 0.0;
-    if (n <= TB) {
+    if constexpr (!BIG) {
       // ---------------- whole state in LDS ----------------
       const int N = 1 << n;
       for (int x = tid; x < N; x += nt) lds[vpad<TB>(x)] = cmk<V>(R(x == 0 ? 1 : 0), R(0));
@@ -217,43 +220,72 @@ __global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, 
           // rotation bits handled by this pass: pass 0 all its bits, others only the chunk
           const int rb_first = ps == 0 ? 0 : kVqeLow;
           const bool last_pass = ps == npass - 1;
-          for (uint64_t t = 0; t < ntp; ++t) {
+          // thread tid owns tile positions tid + 256 k; their offsets inside a tile are
+          // the same for every tile of the pass
+          constexpr int kPer = (1 << TB) / 256;
+          uint32_t off[kPer];
+#pragma unroll
+          for (int k = 0; k < kPer; ++k) {
+            const int p = tid + k * 256;
+            off[k] = p < TNp ? uint32_t(tile_addr(p, hb, nh)) : 0u;
+          }
+          // |0...0> is generated, not read, until the first pass has written the slot
+          const bool gen0 = depth == 0 || (layer == 0 && ps == 0);
+          auto tile_base = [&](uint64_t t) {
             uint64_t tb = t;
             for (int k = 0; k < tbits; ++k) {
               const int pp = lkpos[k];
               tb = ((tb >> pp) << (pp + 1)) | (tb & ((uint64_t(1) << pp) - 1));
             }
-            for (int p = tid; p < TNp; p += nt) {
-              const uint64_t x = tb + tile_addr(p, hb, nh);
-              V v;
-              // |0...0> is generated, not read, until the first pass has written the slot
-              if (depth == 0 || (layer == 0 && ps == 0)) v = cmk<V>(R(x == 0 ? 1 : 0), R(0));
-              else v = nt_load(&slot[x]);
-              lds[vpad<TB>(p)] = v;
+            return tb;
+          };
+          // the next tile's amplitudes are fetched into registers while this tile computes
+          V pre[kPer];
+          auto fetch = [&](uint64_t t) {
+            const uint64_t tbn = tile_base(t);
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+              const int p = tid + k * 256;
+              if (p < TNp) {
+                const uint64_t x = tbn + off[k];
+                pre[k] = gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : nt_load(&slot[x]);
+              }
+            }
+          };
+          fetch(0);
+          for (uint64_t t = 0; t < ntp; ++t) {
+            const uint64_t tb = tile_base(t);
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+              const int p = tid + k * 256;
+              if (p < TNp) lds[vpad<TB>(p)] = pre[k];
             }
             __syncthreads();
+            if (t + 1 < ntp) fetch(t + 1);
             if (layer < depth) {
               for (int g0 = rb_first; g0 < tbits; g0 += 4) {
                 const int gb = min(4, tbits - g0);
                 const bool lastg = g0 + 4 >= tbits;
                 const int nsub = TNp >> gb;
                 // matrices of the group's memory bits
+                // memory bit of each local qubit, made wave-uniform so the 2x2
+                // matrices come in through scalar loads
+                int mbk[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                  const int pos = g0 + k;
+                  const int mb = k < gb ? (pos < kVqeLow ? pos : int(hb[pos - kVqeLow])) : 0;
+                  mbk[k] = __builtin_amdgcn_readfirstlane(mb);
+                }
                 for (int sc = tid; sc < nsub; sc += nt) {
                   const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-                  V a[16], u[16];
-#pragma unroll
-                  for (int k = 0; k < 4; ++k) {
-                    if (k < gb) {
-                      const int pos = g0 + k;
-                      const int mb = pos < kVqeLow ? pos : hb[pos - kVqeLow];
-#pragma unroll
-                      for (int e = 0; e < 4; ++e) u[4 * k + e] = ul[mb * 4 + e];
-                    }
-                  }
+                  V a[16];
 #pragma unroll
                   for (int i = 0; i < 16; ++i)
                     if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
-                  rot_group<R>(a, gb, u);
+#pragma unroll
+                  for (int k = 0; k < 4; ++k)
+                    if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
 #pragma unroll
                   for (int i = 0; i < 16; ++i)
                     if (i < (1 << gb)) {
@@ -268,9 +300,10 @@ __global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, 
                 }
                 __syncthreads();
               }
-              for (int p = tid; p < TNp; p += nt) {
-                const uint64_t x = tb + tile_addr(p, hb, nh);
-                nt_store(lds[vpad<TB>(p)], &slot[x]);
+#pragma unroll
+              for (int k = 0; k < kPer; ++k) {
+                const int p = tid + k * 256;
+                if (p < TNp) nt_store(lds[vpad<TB>(p)], &slot[tb + off[k]]);
               }
             } else {
               // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
@@ -298,7 +331,7 @@ __global__ __launch_bounds__(256) void k_vqe(const VqeInst* __restrict__ insts, 
 }
 
 __global__ void k_reset_counter(int* c) {
-  if (threadIdx.x == 0) *c = 0;
+  if (threadIdx.x < 2) c[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -335,8 +368,9 @@ struct yh_vqe_batch {
   DevBuf insts, rot, scratch, energies, counter;
   VqeConfig cfg;
   bool tuned = false;
+  std::vector<int> sorted_n;  // qubit counts in upload order (descending)
   hipGraphExec_t graph = nullptr;
-  int graph_ninst = -1;
+  int graph_ninst = -1, graph_nbig = -1;
   double graph_J = 0, graph_h = 0;
   VqeConfig graph_cfg;
   ~yh_vqe_batch() {
@@ -351,17 +385,30 @@ static int vqe_launch(yh_vqe_batch* b, int ninst, double J, double h, const VqeC
   using V = typename Cplx<R>::V;
   const int grid = b->cus * cfg.blocks_per_cu;
   const uint64_t slot = uint64_t(1) << b->max_qubits;
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, static_cast<int*>(b->counter.p));
-  if (cfg.tile_bits == 12)
-    hipLaunchKernelGGL((k_vqe<R, 12>), dim3(grid), dim3(cfg.threads), 0, b->w.stream,
-                       static_cast<const VqeInst*>(b->insts.p), ninst, static_cast<int*>(b->counter.p),
-                       static_cast<const V*>(b->rot.p), static_cast<V*>(b->scratch.p), slot,
-                       static_cast<double*>(b->energies.p), J, h);
-  else
-    hipLaunchKernelGGL((k_vqe<R, 13>), dim3(grid), dim3(cfg.threads), 0, b->w.stream,
-                       static_cast<const VqeInst*>(b->insts.p), ninst, static_cast<int*>(b->counter.p),
-                       static_cast<const V*>(b->rot.p), static_cast<V*>(b->scratch.p), slot,
-                       static_cast<double*>(b->energies.p), J, h);
+  // instances are sorted by qubit count, largest first: [0, nbig) stream through HBM
+  int nbig = 0;
+  while (nbig < ninst && b->sorted_n[nbig] > cfg.tile_bits) ++nbig;
+  int* ctr = static_cast<int*>(b->counter.p);
+  const VqeInst* ins = static_cast<const VqeInst*>(b->insts.p);
+  const V* rot = static_cast<const V*>(b->rot.p);
+  V* scr = static_cast<V*>(b->scratch.p);
+  double* en = static_cast<double*>(b->energies.p);
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, ctr);
+  if (cfg.tile_bits == 12) {
+    if (nbig > 0)
+      hipLaunchKernelGGL((k_vqe<R, 12, true>), dim3(grid), dim3(256), 0, b->w.stream, ins, 0, nbig, ctr, rot, scr,
+                         slot, en, J, h);
+    if (nbig < ninst)
+      hipLaunchKernelGGL((k_vqe<R, 12, false>), dim3(grid), dim3(256), 0, b->w.stream, ins, nbig, ninst, ctr + 1,
+                         rot, scr, slot, en, J, h);
+  } else {
+    if (nbig > 0)
+      hipLaunchKernelGGL((k_vqe<R, 13, true>), dim3(grid), dim3(256), 0, b->w.stream, ins, 0, nbig, ctr, rot, scr,
+                         slot, en, J, h);
+    if (nbig < ninst)
+      hipLaunchKernelGGL((k_vqe<R, 13, false>), dim3(grid), dim3(256), 0, b->w.stream, ins, nbig, ninst, ctr + 1,
+                         rot, scr, slot, en, J, h);
+  }
   return YH_OK;
 }
 
@@ -373,8 +420,8 @@ static int vqe_run(yh_vqe_batch* b, int ninst, double J, double h, const VqeConf
 static std::vector<VqeConfig> vqe_candidates(int dtype) {
   std::vector<VqeConfig> c;
   for (int tb : {12, 13}) {
-    if (dtype == YH_C128 && tb == 13) continue;  // 128 KB of LDS: one workgroup per CU
-    for (int bpc : {1, 2, 4})
+    for (int bpc : {1, 2, 4}) {
+      if (dtype == YH_C128 && tb == 13) continue;  // 139 KB of LDS and a 128-VGPR prefetch buffer
       for (int th : {256}) {
         VqeConfig v;
         v.tile_bits = tb;
@@ -382,6 +429,7 @@ static std::vector<VqeConfig> vqe_candidates(int dtype) {
         v.threads = th;
         c.push_back(v);
       }
+    }
   }
   return c;
 }
@@ -445,13 +493,21 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
       }
     if (d == 0) rot.insert(rot.end(), 4, cd{0, 0});  // keep rot_off valid
   }
-  // largest instances first (longest-processing-time order)
+  // qubit count descending (so any tile size splits the table into a streamed prefix
+  // and an LDS-resident suffix), then cost descending (longest-processing-time order)
   std::vector<int> order(ninst);
   std::iota(order.begin(), order.end(), 0);
   auto cost = [&](int i) { return double(int64_t(1) << ins[i].n) * double(ins[i].depth + 1) * ins[i].n; };
-  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost(a) > cost(c); });
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
+    if (ins[a].n != ins[c].n) return ins[a].n > ins[c].n;
+    return cost(a) > cost(c);
+  });
   std::vector<VqeInst> sorted(ninst);
-  for (int k = 0; k < ninst; ++k) sorted[k] = ins[order[k]];
+  b->sorted_n.resize(ninst);
+  for (int k = 0; k < ninst; ++k) {
+    sorted[k] = ins[order[k]];
+    b->sorted_n[k] = sorted[k].n;
+  }
   YH_TRY(b->w.upload(sorted.data(), sorted.size() * sizeof(VqeInst), b->insts));
   if (b->dtype == YH_C128) {
     YH_TRY(b->w.upload(rot.data(), rot.size() * sizeof(cd), b->rot));
@@ -488,8 +544,11 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
     }
     b->tuned = true;
   }
-  // replay a captured graph of (counter reset, persistent kernel) for this shape
-  if (!b->graph || b->graph_ninst != ninst || b->graph_cfg.tile_bits != b->cfg.tile_bits ||
+  // replay a captured graph of (counter reset, persistent kernels) for this shape; the
+  // split between streamed and LDS-resident instances is part of the shape
+  int nbig = 0;
+  while (nbig < ninst && b->sorted_n[nbig] > b->cfg.tile_bits) ++nbig;
+  if (!b->graph || b->graph_ninst != ninst || b->graph_nbig != nbig || b->graph_cfg.tile_bits != b->cfg.tile_bits ||
       b->graph_cfg.blocks_per_cu != b->cfg.blocks_per_cu) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
@@ -504,6 +563,7 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(YH_ERR_HIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
     b->graph_ninst = ninst;
+    b->graph_nbig = nbig;
     b->graph_cfg = b->cfg;
     b->graph_J = J;
     b->graph_h = h;

@@ -66,7 +66,7 @@ int to_tgates(int nbits, const yh_op* ops, size_t nops, std::vector<TGate>& out)
       if (op.kind == YH_GATE_X) {
         g.nt = 1;
         g.tb[0] = bit_of(op.targets[sidx]);
-        g.proto.kind = MG_DENSE1;
+        g.proto.kind = MG_PERM1;
         const double xm[4][2] = {{0, 0}, {1, 0}, {1, 0}, {0, 0}};
         std::memcpy(g.proto.m, xm, sizeof(xm));
       } else {
@@ -148,6 +148,10 @@ void make_groups(const std::vector<TGate>& tg, const std::vector<size_t>& idx, c
       return -1;
     };
     G.first = int(micro.size());
+    // last micro gate touching each local qubit; an uncontrolled 1-qubit gate that
+    // follows an uncontrolled 1-qubit gate on the same qubit is multiplied into it
+    // (FP64, on the host): gates in between act on other qubits and commute.
+    int last_touch[kGroupBits] = {-1, -1, -1, -1};
     for (size_t gi : members) {
       const TGate& g = tg[gi];
       MicroGate mg = g.proto;
@@ -158,7 +162,46 @@ void make_groups(const std::vector<TGate>& tg, const std::vector<size_t>& idx, c
         mg.cmask |= uint8_t(1 << local(g.cb[k]));
         if (g.cv[k]) mg.cval |= uint8_t(1 << local(g.cb[k]));
       }
+      const bool single = mg.kind != MG_DENSE2 && mg.cmask == 0;
+      const int lt = single ? last_touch[mg.t0] : -1;
+      if (lt >= 0 && micro[lt].kind != MG_DENSE2 && micro[lt].cmask == 0 && micro[lt].t0 == mg.t0) {
+        MicroGate& p = micro[lt];
+        cd P[4], Gm[4], M[4];
+        auto dense = [](const MicroGate& x, cd out[4]) {
+          if (x.kind == MG_DIAG1) {
+            out[0] = cd{x.m[0][0], x.m[0][1]};
+            out[1] = out[2] = cd{0, 0};
+            out[3] = cd{x.m[1][0], x.m[1][1]};
+          } else {
+            for (int e = 0; e < 4; ++e) out[e] = cd{x.m[e][0], x.m[e][1]};
+          }
+        };
+        dense(p, P);
+        dense(mg, Gm);
+        for (int r = 0; r < 2; ++r)
+          for (int c = 0; c < 2; ++c) M[r * 2 + c] = Gm[r * 2] * P[c] + Gm[r * 2 + 1] * P[2 + c];
+        const bool diag = M[1].re == 0 && M[1].im == 0 && M[2].re == 0 && M[2].im == 0;
+        if (diag) {
+          p.kind = MG_DIAG1;
+          p.m[0][0] = M[0].re;
+          p.m[0][1] = M[0].im;
+          p.m[1][0] = M[3].re;
+          p.m[1][1] = M[3].im;
+        } else {
+          p.kind = MG_DENSE1;
+          for (int e = 0; e < 4; ++e) {
+            p.m[e][0] = M[e].re;
+            p.m[e][1] = M[e].im;
+          }
+        }
+        continue;
+      }
       micro.push_back(mg);
+      const int idx = int(micro.size()) - 1;
+      last_touch[mg.t0] = idx;
+      if (mg.kind == MG_DENSE2) last_touch[mg.t1] = idx;
+      for (int k = 0; k < kGroupBits; ++k)
+        if ((mg.cmask >> k) & 1) last_touch[k] = idx;
     }
     G.count = int(micro.size()) - G.first;
     groups.push_back(G);
@@ -290,7 +333,7 @@ int plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]) {
   out[0] = out[3] = 0;
   for (auto& s : steps) (s.tile ? out[0] : out[3]) += 1;
   out[1] = int64_t(groups.size());
-  out[2] = int64_t(micro.size());
+  out[2] = int64_t(tg.size()) - out[3];  // gates (X split per target) placed in tiles
   return YH_OK;
 }
 

@@ -18,7 +18,7 @@ constexpr int kAmpsPerThread = kTileN / kTileThreads;
 
 // One gate inside a group, in the group's local index space (local qubit k <-> tile
 // qubit Group::q[k]). Gate-index bit 0 <-> t0, bit 1 <-> t1 (yao order).
-enum MicroKind : uint8_t { MG_DENSE1 = 0, MG_DIAG1 = 1, MG_DENSE2 = 2 };
+enum MicroKind : uint8_t { MG_DENSE1 = 0, MG_DIAG1 = 1, MG_DENSE2 = 2, MG_PERM1 = 3 /* X: pair swap */ };
 struct MicroGate {
   uint8_t kind, t0, t1, cmask, cval, pad[3];
   double m[16][2];  // dense1: 4 entries, diag1: 2, dense2: 16 (re, im)

@@ -41,6 +41,21 @@ __device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const Mi
 }
 
 template <class R, int T>
+__device__ __forceinline__ void mg_perm1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
+  const int cm = g.cmask, cv = g.cval;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (i & (1 << T)) continue;
+    const int j = i | (1 << T);
+    if ((i & cm) == cv) {
+      const typename Cplx<R>::V w = a[i];
+      a[i] = a[j];
+      a[j] = w;
+    }
+  }
+}
+
+template <class R, int T>
 __device__ __forceinline__ void mg_diag1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
   using V = typename Cplx<R>::V;
   const V d0 = ldm<R>(g.m[0]), d1 = ldm<R>(g.m[1]);
@@ -92,6 +107,14 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
         case 1: mg_diag1<R, 1>(a, g); break;
         case 2: mg_diag1<R, 2>(a, g); break;
         default: mg_diag1<R, 3>(a, g); break;
+      }
+      break;
+    case MG_PERM1:
+      switch (g.t0) {
+        case 0: mg_perm1<R, 0>(a, g); break;
+        case 1: mg_perm1<R, 1>(a, g); break;
+        case 2: mg_perm1<R, 2>(a, g); break;
+        default: mg_perm1<R, 3>(a, g); break;
       }
       break;
     default:

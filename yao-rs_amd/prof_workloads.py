@@ -1,0 +1,61 @@
+"""Named workloads for rocprofv3 runs (kernel traces and PMC counters).
+
+usage: python yao-rs_amd/prof_workloads.py WORKLOAD [nbits] [depth]
+  tiled      random brickwork through yh_apply_fused (one step)
+  gbg        the same circuit gate by gate (one step)
+  copy       one device-to-device copy of a 2^nbits complex128 state (the
+             reference kernel for scaling FETCH_SIZE / WRITE_SIZE)
+  vqe        one batch of 4096 ansatz instances (n in 10..20, depth 4..8), complex64
+  dm         noisy brickwork on a density matrix (fused gate+channel passes)
+Each workload prints one line naming itself before it touches the GPU.
+"""
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path[:0] = [HERE, os.path.join(ROOT, "tests")]
+
+import numpy as np  # noqa: E402
+
+
+def main():
+    w = sys.argv[1]
+    n = int(sys.argv[2]) if len(sys.argv) > 2 else 28
+    depth = int(sys.argv[3]) if len(sys.argv) > 3 else 20
+    print(f"[prof] workload={w} nbits={n} depth={depth}", flush=True)
+    import yaohip
+    from yaohip.circuit import Circuit
+    from fixture_circuits import brickwork, noisy_brickwork
+    if w in ("tiled", "gbg"):
+        reg = yaohip.ArrayReg.zero_state(n)
+        circ = Circuit.qubits(n, brickwork(n, depth, 20))
+        t0 = time.perf_counter()
+        yaohip.apply_inplace(circ, reg, 4 if w == "tiled" else 0)
+        print(f"[prof] {w} {time.perf_counter() - t0:.3f} s norm {reg.norm():.15f}", flush=True)
+    elif w == "copy":
+        a = yaohip.ArrayReg.zero_state(n)
+        b = yaohip.ArrayReg.zero_state(n)
+        from yaohip import _abi as A
+        for _ in range(3):
+            A.check(A.lib().yh_reg_copy(b._h, a._h))
+        print(f"[prof] copy of {16 << n} bytes x3 done", flush=True)
+    elif w == "vqe":
+        from yaohip.vqe import VqeBatch, random_instances
+        inst = random_instances(4096, seed=4)
+        b = VqeBatch(4096, 20, "c64")
+        e = b.energies(inst)
+        t0 = time.perf_counter()
+        e = b.energies(inst)
+        print(f"[prof] vqe batch {time.perf_counter() - t0:.4f} s mean E {e.mean():.6f}", flush=True)
+    elif w == "dm":
+        dm = yaohip.DensityMatrix.zero_state(n)
+        dm.apply(Circuit.qubits(n, noisy_brickwork(n, depth, 3)), fused=True)
+        print(f"[prof] dm trace {dm.trace().real:.15f}", flush=True)
+    else:
+        raise SystemExit(f"unknown workload {w}")
+
+
+if __name__ == "__main__":
+    main()
