@@ -219,10 +219,12 @@ int yh_dm_timing_read(const yh_dm *dm, int kclass, uint64_t *launches, double *t
 
 /* ---- batched VQE (no reference counterpart: many ansatz instances per launch) ---- */
 typedef struct yh_vqe_batch yh_vqe_batch;
-/* Hardware-efficient ansatz: `depth` layers of Ry then Rz on every qubit, then a CZ
- * chain CZ(i, i+1), i < n-1 (CZ = control([i],[i+1],Z), easybuild.rs:516), from
- * |0...0>. Instance b has nq[b] qubits and depth[b] layers; its 2*nq*depth angles
- * are angles[offset[b] ...] ordered layer, qubit, (Ry, Rz).
+/* Hardware-efficient ansatz from |0...0>: `depth` layers, each a single-qubit block on
+ * every qubit followed by a CZ chain CZ(i, i+1), i < n-1 (CZ = control([i],[i+1],Z),
+ * easybuild.rs:516). angles_per_qubit = 2: the block is Ry then Rz; 3: Rz, Ry, Rz
+ * (general_u2, easybuild.rs:75-81). Instance b has nq[b] qubits and depth[b] layers;
+ * its angles_per_qubit*nq*depth angles are angles[offset[b] ...] ordered layer,
+ * qubit, then the block's gates in application order.
  * Energy: TFIM H = -J sum Z_i Z_{i+1} - h sum X_i, open boundary.
  * dtype YH_C64 keeps amplitudes in complex64 and reduces in FP64; YH_C128 is the
  * all-FP64 path. */
@@ -230,12 +232,39 @@ int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_bat
 int yh_vqe_batch_destroy(yh_vqe_batch *b);
 int yh_vqe_batch_energies(yh_vqe_batch *b, int ninst, const int32_t *nq,
                           const int32_t *depth, const int64_t *offset, const double *angles,
-                          double J, double h, double *energies);
+                          int angles_per_qubit, double J, double h, double *energies);
 /* Same circuits, evaluated one instance per launch sequence (the plain per-circuit
  * GPU path through yh_apply + yh_expect), for comparison. */
 int yh_vqe_plain_energies(int ninst, const int32_t *nq, const int32_t *depth,
-                          const int64_t *offset, const double *angles, double J, double h,
-                          int dtype, double *energies);
+                          const int64_t *offset, const double *angles, int angles_per_qubit,
+                          double J, double h, int dtype, double *energies);
+
+/* ---- MLP surrogate (no reference counterpart) ----
+ * in_dim -> `hidden` layers of `width` (GELU) -> 1, y = x W^T + b per layer. FP32
+ * master parameters; forward and backward run BF16 MFMA GEMMs with FP32
+ * accumulation. Parameters are flat FP32, layer by layer: W (width x fan_in,
+ * row-major) then b. Input matrices are row-major FP32 (n x in_dim).
+ * Input encodings: YH_MLP_LINEAR feeds x as is (fan_in of layer 0 = in_dim,
+ * in_dim % 16 == 0); YH_MLP_TRIG feeds (cos x_j, sin x_j) at features 2j, 2j+1
+ * (fan_in = 2 in_dim, in_dim % 8 == 0), for inputs that are rotation angles.
+ * width % 64 == 0, 1 <= hidden <= 16, max_batch % 64 == 0. */
+typedef struct yh_mlp yh_mlp;
+typedef enum { YH_MLP_LINEAR = 0, YH_MLP_TRIG = 1 } yh_mlp_encoding;
+int yh_mlp_create(int in_dim, int encoding, int width, int hidden, int max_batch, uint64_t seed, yh_mlp **out);
+int yh_mlp_destroy(yh_mlp *m);
+int yh_mlp_num_params(const yh_mlp *m, size_t *n);
+int yh_mlp_get_params(const yh_mlp *m, float *params, size_t n);
+int yh_mlp_set_params(yh_mlp *m, const float *params, size_t n);
+/* y = f(x) for n rows (any n; processed in chunks of max_batch). */
+int yh_mlp_predict(yh_mlp *m, const float *x, size_t n, float *y);
+/* Mean-squared-error loss of one batch (n <= max_batch) and its gradient with
+ * respect to every parameter (grads may be NULL), without updating. */
+int yh_mlp_loss_grad(yh_mlp *m, const float *x, const float *y, int n, float *loss, float *grads);
+/* Adam training: the n rows are uploaded once; each epoch visits them in a seeded
+ * random order in batches of `batch` rows. epoch_loss[e] receives the mean batch
+ * loss of epoch e (may be NULL). */
+int yh_mlp_train(yh_mlp *m, const float *x, const float *y, size_t n, int batch, int epochs, float lr,
+                 uint64_t seed, float *epoch_loss);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,9 @@ def lib():
             "orc_dm_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
             "orc_dm_partial_tr": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d),
                                                  P(ctypes.c_int)]),
+            "orc_mlp_loss_grad": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(d),
+                                                 P(ctypes.c_float), P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                                 P(d), P(d), P(d)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -247,3 +250,27 @@ def dm_partial_tr(nbits, rho, traced):
                                 _d(out), ctypes.byref(kept)))
     k = kept.value
     return _cx(out[:2 << (2 * k)]).reshape(1 << k, 1 << k)
+
+
+def mlp_num_params(in_dim, width, hidden, trig=False):
+    f = 2 * in_dim if trig else in_dim
+    return width * (f + 1) + (hidden - 1) * width * (width + 1) + width + 1
+
+
+def mlp_loss_grad(in_dim, width, hidden, params, x, y, quant=True, trig=False):
+    """FP64 forward/backward of the surrogate MLP; returns (predictions, loss, grads).
+    quant rounds to bfloat16 where the GPU stores bfloat16 operands; trig feeds
+    (cos, sin) pairs of the inputs."""
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    assert params.size == mlp_num_params(in_dim, width, hidden, trig)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    n = y.size
+    assert x.size == n * in_dim
+    out = np.zeros(n)
+    loss = np.zeros(1)
+    grads = np.zeros(params.size)
+    fp = ctypes.POINTER(ctypes.c_float)
+    _ck(lib().orc_mlp_loss_grad(in_dim, int(bool(trig)), width, hidden, _d(params), x.ctypes.data_as(fp),
+                                y.ctypes.data_as(fp), n, int(bool(quant)), _d(out), _d(loss), _d(grads)))
+    return out, float(loss[0]), grads

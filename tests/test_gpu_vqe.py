@@ -60,3 +60,29 @@ def test_vqe_batch_repeat_and_coefficients():
     e3 = b.energies(inst, J=0.5, h=2.0)
     want = plain_energies(inst, J=0.5, h=2.0, dtype="c128")
     assert np.max(np.abs(e3 - want)) < 1e-10
+
+
+def test_vqe_batch_rz_ry_rz_block_vs_oracle():
+    """Three angles per qubit per layer (Rz, Ry, Rz), small instances against the oracle."""
+    inst = random_instances(24, seed=12, nmin=2, nmax=13, dmin=0, dmax=5, apq=3)
+    want = oracle_energies(inst)
+    got = VqeBatch(64, 20, "c128").energies(inst)
+    assert np.max(np.abs(got - want)) < 1e-10
+
+
+def test_vqe_batch_rz_ry_rz_large_vs_plain():
+    inst = random_instances(8, seed=13, nmin=16, nmax=18, dmin=3, dmax=6, apq=3)
+    plain = plain_energies(inst, dtype="c128")
+    got = VqeBatch(64, 20, "c128").energies(inst)
+    assert np.max(np.abs(got - plain)) < 1e-10
+    got64 = VqeBatch(64, 20, "c64").energies(inst)
+    assert np.max(np.abs(got64 - plain)) < 1e-4
+
+
+def test_vqe_batch_rejects_bad_angle_count():
+    import yaohip
+    inst = random_instances(2, seed=14, nmin=4, nmax=4, dmin=1, dmax=1)
+    b = VqeBatch(8, 20, "c128")
+    inst.apq = 4
+    with pytest.raises(yaohip.YhError):
+        b.energies(inst)

@@ -336,19 +336,30 @@ __global__ void k_reset_counter(int* c) {
 
 }  // namespace
 
-// Composed per-qubit layer matrix U = Rz(tz) * Ry(ty) (Ry first), FP64.
-static void rzry(double ty, double tz, cd u[4]) {
-  Mat ry, rz;
-  yh_op o{};
-  o.elem = YH_ELEM_GATE;
-  o.kind = YH_GATE_RY;
-  o.params[0] = ty;
-  gate_matrix(o, ry);
-  o.kind = YH_GATE_RZ;
-  o.params[0] = tz;
-  gate_matrix(o, rz);
-  for (int r = 0; r < 2; ++r)
-    for (int c = 0; c < 2; ++c) u[r * 2 + c] = rz[r * 2 + 0] * ry[0 * 2 + c] + rz[r * 2 + 1] * ry[1 * 2 + c];
+// The gate kinds of one qubit's block per layer, in application order.
+static const int* block_kinds(int apq) {
+  static const int ryrz[2] = {YH_GATE_RY, YH_GATE_RZ};
+  static const int u2[3] = {YH_GATE_RZ, YH_GATE_RY, YH_GATE_RZ};  // general_u2 (easybuild.rs:75-81)
+  return apq == 3 ? u2 : ryrz;
+}
+
+// Composed per-qubit layer matrix U = G_last * ... * G_first, FP64.
+static void compose_block(const double* a, int apq, cd u[4]) {
+  const int* kinds = block_kinds(apq);
+  cd acc[4] = {cd{1, 0}, cd{0, 0}, cd{0, 0}, cd{1, 0}};
+  for (int g = 0; g < apq; ++g) {
+    Mat m;
+    yh_op o{};
+    o.elem = YH_ELEM_GATE;
+    o.kind = kinds[g];
+    o.params[0] = a[g];
+    gate_matrix(o, m);
+    cd nxt[4];
+    for (int r = 0; r < 2; ++r)
+      for (int c = 0; c < 2; ++c) nxt[r * 2 + c] = m[r * 2 + 0] * acc[0 * 2 + c] + m[r * 2 + 1] * acc[1 * 2 + c];
+    for (int e = 0; e < 4; ++e) acc[e] = nxt[e];
+  }
+  for (int e = 0; e < 4; ++e) u[e] = acc[e];
 }
 
 struct VqeConfig {
@@ -473,7 +484,7 @@ int yh_vqe_batch_destroy(yh_vqe_batch* b) {
 
 // Uploads the instance table and the FP64-composed layer matrices.
 static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
-                      const double* angles) {
+                      const double* angles, int apq) {
   std::vector<VqeInst> ins(ninst);
   std::vector<cd> rot;
   for (int i = 0; i < ninst; ++i) {
@@ -486,9 +497,9 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
     for (int l = 0; l < d; ++l)
       for (int bit = 0; bit < n; ++bit) {
         const int loc = n - 1 - bit;
-        const double* a = angles + offset[i] + (int64_t(l) * n + loc) * 2;
+        const double* a = angles + offset[i] + (int64_t(l) * n + loc) * apq;
         cd u[4];
-        rzry(a[0], a[1], u);
+        compose_block(a, apq, u);
         rot.insert(rot.end(), u, u + 4);
       }
     if (d == 0) rot.insert(rot.end(), 4, cd{0, 0});  // keep rot_off valid
@@ -523,10 +534,11 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
 }
 
 int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
-                          const double* angles, double J, double h, double* energies) {
+                          const double* angles, int apq, double J, double h, double* energies) {
+  if (apq != 2 && apq != 3) return fail(YH_ERR_INVALID_ARG, "angles_per_qubit must be 2 or 3");
   if (!b || !nq || !depth || !offset || !angles || !energies) return fail(YH_ERR_INVALID_ARG, "null argument");
   if (ninst < 1 || ninst > b->max_instances) return fail(YH_ERR_INVALID_ARG, "instance count out of range");
-  YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles));
+  YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles, apq));
   if (!b->tuned) {
     // autotune: time every candidate launch shape on this batch, keep the fastest
     double best = 1e30;
@@ -581,8 +593,10 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
 // The plain per-circuit GPU path: every instance through yh_apply (one launch per
 // gate) and yh_expect, as a caller of the drop-in API would evaluate it.
 int yh_vqe_plain_energies(int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
-                          const double* angles, double J, double h, int dtype, double* energies) {
+                          const double* angles, int apq, double J, double h, int dtype, double* energies) {
   if (!nq || !depth || !offset || !angles || !energies) return fail(YH_ERR_INVALID_ARG, "null argument");
+  if (apq != 2 && apq != 3) return fail(YH_ERR_INVALID_ARG, "angles_per_qubit must be 2 or 3");
+  const int* kinds = block_kinds(apq);
   std::vector<yh_reg*> regs(40, nullptr);
   int st = YH_OK;
   for (int i = 0; i < ninst && st == YH_OK; ++i) {
@@ -603,17 +617,16 @@ int yh_vqe_plain_energies(int ninst, const int32_t* nq, const int32_t* depth, co
     std::vector<yh_op> ops;
     for (int l = 0; l < d; ++l) {
       for (int q = 0; q < n; ++q) {
-        const double* a = angles + offset[i] + (int64_t(l) * n + q) * 2;
+        const double* a = angles + offset[i] + (int64_t(l) * n + q) * apq;
         yh_op o{};
         o.elem = YH_ELEM_GATE;
         o.nt = 1;
         o.targets[0] = q;
-        o.kind = YH_GATE_RY;
-        o.params[0] = a[0];
-        ops.push_back(o);
-        o.kind = YH_GATE_RZ;
-        o.params[0] = a[1];
-        ops.push_back(o);
+        for (int g = 0; g < apq; ++g) {
+          o.kind = kinds[g];
+          o.params[0] = a[g];
+          ops.push_back(o);
+        }
       }
       for (int q = 0; q + 1 < n; ++q) {
         yh_op o{};

@@ -29,6 +29,7 @@ GATE_NAMES = ["X", "Y", "Z", "H", "S", "T", "SWAP", "Phase", "Rx", "Ry", "Rz", "
  CH_PHASE_DAMPING, CH_PHASE_AMPLITUDE_DAMPING, CH_THERMAL_RELAXATION, CH_COHERENT,
  CH_CUSTOM) = range(11)
 OP_I, OP_X, OP_Y, OP_Z, OP_P0, OP_P1, OP_PU, OP_PD = range(8)
+MLP_LINEAR, MLP_TRIG = 0, 1
 (K_1Q, K_DENSE, K_DIAG, K_X, K_SWAP, K_OOP, K_FUSED, K_TILE, K_DM, K_VQE) = range(10)
 KCLASS_NAMES = ["1q", "dense", "diag", "x", "swap", "oop", "fused", "tile", "dm", "vqe"]
 MAX_TARGETS = 12
@@ -158,11 +159,21 @@ SIGNATURES = [
     ("yh_vqe_batch_create", c_int, [c_int, c_int, c_int, P(c_void_p)]),
     ("yh_vqe_batch_destroy", c_int, [c_void_p]),
     ("yh_vqe_batch_energies", c_int, [c_void_p, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
-                                      P(ctypes.c_int64), P(c_double), c_double, c_double,
+                                      P(ctypes.c_int64), P(c_double), c_int, c_double, c_double,
                                       P(c_double)]),
     ("yh_vqe_plain_energies", c_int, [c_int, P(ctypes.c_int32), P(ctypes.c_int32),
-                                      P(ctypes.c_int64), P(c_double), c_double, c_double, c_int,
+                                      P(ctypes.c_int64), P(c_double), c_int, c_double, c_double, c_int,
                                       P(c_double)]),
+    ("yh_mlp_create", c_int, [c_int, c_int, c_int, c_int, c_int, ctypes.c_uint64, P(c_void_p)]),
+    ("yh_mlp_destroy", c_int, [c_void_p]),
+    ("yh_mlp_num_params", c_int, [c_void_p, P(c_size_t)]),
+    ("yh_mlp_get_params", c_int, [c_void_p, P(ctypes.c_float), c_size_t]),
+    ("yh_mlp_set_params", c_int, [c_void_p, P(ctypes.c_float), c_size_t]),
+    ("yh_mlp_predict", c_int, [c_void_p, P(ctypes.c_float), c_size_t, P(ctypes.c_float)]),
+    ("yh_mlp_loss_grad", c_int, [c_void_p, P(ctypes.c_float), P(ctypes.c_float), c_int, P(ctypes.c_float),
+                                 P(ctypes.c_float)]),
+    ("yh_mlp_train", c_int, [c_void_p, P(ctypes.c_float), P(ctypes.c_float), c_size_t, c_int, c_int,
+                             ctypes.c_float, ctypes.c_uint64, P(ctypes.c_float)]),
 ]
 
 

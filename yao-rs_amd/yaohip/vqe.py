@@ -1,7 +1,8 @@
 """Batched VQE energies over the C ABI (yh_vqe_batch_*), plus the workload
-generator: hardware-efficient ansatz instances (per layer Ry then Rz on every
-qubit, then a CZ chain CZ(i, i+1)) on the open transverse-field Ising chain
-H = -J sum Z_i Z_{i+1} - h sum X_i."""
+generator: hardware-efficient ansatz instances (per layer a single-qubit block on
+every qubit, then a CZ chain CZ(i, i+1)) on the open transverse-field Ising chain
+H = -J sum Z_i Z_{i+1} - h sum X_i. The block is Ry then Rz (2 angles per qubit)
+or Rz, Ry, Rz (general_u2, 3 angles per qubit)."""
 import ctypes
 import math
 
@@ -10,12 +11,15 @@ import numpy as np
 from . import _abi as A
 from .circuit import Gate, control, put, tfim_hamiltonian
 
+BLOCKS = {2: ("Ry", "Rz"), 3: ("Rz", "Ry", "Rz")}
+
 
 class VqeInstances:
     """Instance table: qubit counts, depths, angle offsets and the flat angle array,
-    ordered per instance as (layer, loc, [Ry, Rz])."""
+    ordered per instance as (layer, loc, block gate)."""
 
-    def __init__(self, nq, depth, angles_per_instance):
+    def __init__(self, nq, depth, angles_per_instance, apq=2):
+        self.apq = int(apq)
         self.nq = np.ascontiguousarray(nq, dtype=np.int32)
         self.depth = np.ascontiguousarray(depth, dtype=np.int32)
         offs, flat = [], []
@@ -30,29 +34,47 @@ class VqeInstances:
 
     def instance_angles(self, i):
         n, d = int(self.nq[i]), int(self.depth[i])
-        return self.angles[self.offset[i]: self.offset[i] + 2 * n * d].reshape(d, n, 2)
+        return self.angles[self.offset[i]: self.offset[i] + self.apq * n * d].reshape(d, n, self.apq)
+
+    def subset(self, idx):
+        return VqeInstances(self.nq[idx], self.depth[idx], [self.instance_angles(i) for i in idx], self.apq)
 
     def circuit_elements(self, i):
         """The instance as a yao-rs element list (CZ = control([i],[i+1], Z))."""
         n, d = int(self.nq[i]), int(self.depth[i])
         a = self.instance_angles(i)
+        mk = {"Rx": Gate.Rx, "Ry": Gate.Ry, "Rz": Gate.Rz}
         els = []
         for l in range(d):
             for q in range(n):
-                els.append(put([q], Gate.Ry(a[l, q, 0])))
-                els.append(put([q], Gate.Rz(a[l, q, 1])))
+                for g, name in enumerate(BLOCKS[self.apq]):
+                    els.append(put([q], mk[name](a[l, q, g])))
             for q in range(n - 1):
                 els.append(control([q], [q + 1], Gate.Z))
         return els
 
 
-def random_instances(count, seed, nmin=10, nmax=20, dmin=4, dmax=8, amin=-math.pi, amax=math.pi):
+def random_instances(count, seed, nmin=10, nmax=20, dmin=4, dmax=8, amin=-math.pi, amax=math.pi, apq=2):
     """Seeded instances: n ~ U{nmin..nmax}, depth ~ U{dmin..dmax}, angles ~ U[amin, amax]."""
     rng = np.random.default_rng(seed)
     nq = rng.integers(nmin, nmax + 1, size=count)
     depth = rng.integers(dmin, dmax + 1, size=count)
-    angles = [rng.uniform(amin, amax, size=2 * int(n) * int(d)) for n, d in zip(nq, depth)]
-    return VqeInstances(nq, depth, angles)
+    angles = [rng.uniform(amin, amax, size=apq * int(n) * int(d)) for n, d in zip(nq, depth)]
+    return VqeInstances(nq, depth, angles, apq)
+
+
+def fixed_instances(angles, n, depth, apq):
+    """Instances that share n and depth; angles is (count, depth*n*apq)."""
+    angles = np.asarray(angles, dtype=np.float64)
+    c = angles.shape[0]
+    inst = VqeInstances.__new__(VqeInstances)
+    inst.apq = apq
+    inst.nq = np.full(c, n, dtype=np.int32)
+    inst.depth = np.full(c, depth, dtype=np.int32)
+    inst.offset = np.ascontiguousarray(np.arange(c, dtype=np.int64) * angles.shape[1])
+    inst.angles = np.ascontiguousarray(angles.reshape(-1))
+    inst.count = c
+    return inst
 
 
 class VqeBatch:
@@ -73,7 +95,8 @@ class VqeBatch:
     def energies(self, inst: VqeInstances, J=1.0, h=1.0):
         out = np.zeros(inst.count)
         A.check(A.lib().yh_vqe_batch_energies(self._h, inst.count, A.i32ptr(inst.nq), A.i32ptr(inst.depth),
-                                              A.i64ptr(inst.offset), A.dptr(inst.angles), J, h, A.dptr(out)))
+                                              A.i64ptr(inst.offset), A.dptr(inst.angles), inst.apq, J, h,
+                                              A.dptr(out)))
         return out
 
 
@@ -82,7 +105,7 @@ def plain_energies(inst: VqeInstances, J=1.0, h=1.0, dtype="c128"):
     A.init()
     out = np.zeros(inst.count)
     A.check(A.lib().yh_vqe_plain_energies(inst.count, A.i32ptr(inst.nq), A.i32ptr(inst.depth), A.i64ptr(inst.offset),
-                                          A.dptr(inst.angles), J, h, A.C64 if dtype == "c64" else A.C128,
+                                          A.dptr(inst.angles), inst.apq, J, h, A.C64 if dtype == "c64" else A.C128,
                                           A.dptr(out)))
     return out
 
