@@ -1,10 +1,13 @@
 """Benchmark: 28-qubit random brickwork circuit (depth 20) on a complex128 state.
 
 A step applies the whole circuit (1,950 gates: Rx, Ry, Rz on every qubit per layer,
-then CNOTs on even / odd pairs) to the 4 GiB state resident in HBM. Prints one JSON
-line with whole-job gate applications per second, the dominant kernel's roofline
-(live HIP-event timing on the library's stream) and the CPU oracle timed on a
-bounded sample of the same workload on this host.
+then CNOTs on even / odd pairs) to the 4 GiB state resident in HBM, as 5-qubit
+fused blocks applied with FP64 MFMA (yh_apply_fused, max_fuse_qubits = 5). Prints
+one JSON line with whole-job gate applications per second, the dominant kernel's
+roofline (live HIP-event timing on the library's stream) and the CPU oracle timed
+on a bounded sample of the same workload on this host. The same circuit is also
+timed gate by gate and through LDS tile passes, and the other workloads (complex64,
+batched VQE, noisy density matrix, learned surrogate) are reported beside it.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -24,6 +27,14 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+FP64_MFMA_PEAK_TFLOPS = 78.6
+FP32_MFMA_PEAK_TFLOPS = 157.3
+# HBM bytes per launch of k_block5<double> on the 28-qubit state, from rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 72 launches
+# of one step), each scaled by the factor a plain copy of known size gives for the
+# same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
+# profiles/k_block5_brickwork28_pmc.md.
+BLOCK28_TRAFFIC_BYTES = int((2 * 2101040 + 4244230) * 1024)
 NBITS = 28
 DEPTH = 20
 SEED = 20
@@ -129,30 +140,49 @@ def main():
         log(f"{label}: {dt / steps * 1e3:.2f} ms/step; norm {reg.norm():.15f}; kernels {kstats}")
         return dt, kstats
 
-    def roofline_of(kstats):
-        # dominant kernel; either pass type reads and writes every amplitude once: 2 x 16 B
+    def roofline_of(kstats, amp_bytes=16):
+        # dominant kernel; every pass type reads and writes every amplitude once
         dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
-        bytes_per_launch = 2 * 16 * (1 << n)
+        bytes_per_launch = 2 * amp_bytes * (1 << n)
         achieved = bytes_per_launch / (kstats[dom]["avg_us"] * 1e-6) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(kstats[dom]["avg_us"], 2)}
 
+    def mfma_of(kstats, peak):
+        # real-split 64 x 64 product per 32 amplitudes: 64 * 64 * 2 flops per 32 amplitudes
+        flops = (1 << n) * 64 * 64 * 2 // 32
+        t = kstats["fused"]["avg_us"] * 1e-6
+        return {"flops_per_launch": flops, "achieved_tflops": round(flops / t / 1e12, 2), "peak_tflops": peak,
+                "frac": round(flops / t / 1e12 / peak, 4)}
+
     gbg_dt, gbg_k = run(0, 1, 0 if args.warmup == 0 else 1, "gate-by-gate")
-    dt, kstats = run(4, args.steps, args.warmup, "tiled")
+    tile_dt, tile_k = run(4, 1, 0 if args.warmup == 0 else 1, "tiled")
+    dt, kstats = run(5, args.steps, args.warmup, "blocks")
+    bplan = np.zeros(3, dtype=np.int64)
+    A.check(L.yh_block_plan_stats(n, ol.arr, ol.n, A.i64ptr(bplan)))
     value = ngates * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
+    roof = roofline_of(kstats)
+    if n == 28:
+        roof["traffic"] = BLOCK28_TRAFFIC_BYTES
     out = {
         "metric": "gate-apps/s at 28q c128 (random brickwork depth 20)",
         "value": round(value, 2), "unit": "gate-apps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"random brickwork {n}q depth {depth} complex128 from |0...0>, multi-gate "
-                               f"tile passes (yh_apply_fused)", "nbits": n, "depth": depth, "gates": ngates,
-                   "tile_passes": int(plan[0]), "gate_groups": int(plan[1])},
-        "hbm_gbps_effective": round(int(plan[0]) * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
+        "config": {"workload": f"random brickwork {n}q depth {depth} complex128 from |0...0>, 5-qubit fused "
+                               f"blocks applied with FP64 MFMA (yh_apply_fused, max_fuse_qubits=5)",
+                   "nbits": n, "depth": depth, "gates": ngates, "blocks": int(bplan[0])},
+        "hbm_gbps_effective": round(int(bplan[0]) * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
         "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
-        "roofline": roofline_of(kstats),
+        "roofline": roof,
+        "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS),
+        "tiled": {"value": round(ngates / tile_dt, 2), "unit": "gate-apps/s", "ms_per_step": round(1e3 * tile_dt, 3),
+                  "tile_passes": int(plan[0]), "gate_groups": int(plan[1]),
+                  "hbm_gbps_effective": round(int(plan[0]) * 2 * 16 * (1 << n) / tile_dt / 1e9, 1),
+                  "roofline": roofline_of(tile_k),
+                  "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in tile_k.items()}},
         "gate_by_gate": {"value": round(ngates / gbg_dt, 2), "unit": "gate-apps/s",
                          "ms_per_step": round(1e3 * gbg_dt, 3),
                          "hbm_gbps_effective": round(step_bytes / gbg_dt / 1e9, 1),
@@ -164,24 +194,29 @@ def main():
     if not args.skip_extra:
         del reg
         reg = yaohip.ArrayReg.zero_state(n)
-        yaohip.apply_inplace(circ, reg, 4)
+        yaohip.apply_inplace(circ, reg, 5)
         ref = reg.state_vec()
         reg64 = yaohip.ArrayReg.zero_state(n, "c64")
-        yaohip.apply_inplace(circ, reg64, 4)
+        yaohip.apply_inplace(circ, reg64, 5)
+        A.check(L.yh_reg_timing_enable(reg64._h, 1))
         t0 = time.perf_counter()
         steps64 = max(1, args.steps)
         for _ in range(steps64):
-            yaohip.apply_inplace(circ, reg64, 4)
+            yaohip.apply_inplace(circ, reg64, 5)
         dt64 = time.perf_counter() - t0
+        c, ms = ctypes.c_uint64(), ctypes.c_double()
+        A.check(L.yh_reg_timing_read(reg64._h, A.K_FUSED, ctypes.byref(c), ctypes.byref(ms)))
+        k64 = {"fused": {"launches": c.value, "avg_us": 1e3 * ms.value / max(1, c.value), "total_ms": ms.value}}
         reg64b = yaohip.ArrayReg.zero_state(n, "c64")
-        yaohip.apply_inplace(circ, reg64b, 4)
+        yaohip.apply_inplace(circ, reg64b, 5)
         got64 = reg64b.state_vec()
         err64 = float(np.max(np.abs(got64 - ref)) / np.max(np.abs(ref)))
         fid64 = float(abs(np.vdot(ref, got64)) ** 2)
         out["complex64"] = {"value": round(ngates * steps64 / dt64, 2), "unit": "gate-apps/s",
                             "ms_per_step": round(1e3 * dt64 / steps64, 3),
                             "max_rel_err_vs_fp64": err64, "fidelity_vs_fp64": fid64,
-                            "dtype": "f32 amplitudes, f64 host-fused matrices"}
+                            "roofline": roofline_of(k64, amp_bytes=8), "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS),
+                            "dtype": "f32 amplitudes and FP32 MFMA, block matrices fused in f64 on the host"}
         log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
         del reg64, reg64b, got64, ref
         out["vqe_batched"] = bench_vqe(log)

@@ -131,11 +131,18 @@ int yh_validate(int nbits, const yh_op *ops, size_t nops);
  * skipped for a pure register. One kernel launch per gate (dispatch_arrayreg_gate
  * :7-120). */
 int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
-/* Same semantics as yh_apply; gates are packed into passes that each stream the
- * state through HBM once (gate runs on up to `max_fuse_qubits` qubits fused into
- * one 2^k x 2^k block, applied with MFMA). max_fuse_qubits in [1,5]; 0 = default. */
+/* Same semantics as yh_apply (apply_inplace, apply.rs:123-147), with gates packed
+ * into passes that each stream the state through HBM once.
+ * max_fuse_qubits = 5: gates fused into 5-qubit blocks (32 x 32 complex matrices,
+ *   FP64 on the host), each applied as a real-split MFMA contraction (FP64 MFMA
+ *   for complex128, FP32 MFMA for complex64).
+ * max_fuse_qubits in 0..4: multi-gate LDS tile passes (12 qubits per pass, gates
+ *   applied in groups of up to 4 qubits; 0 = this default). */
 int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
                    int max_fuse_qubits);
+/* Host-only plan of the 5-qubit block path: out[0] blocks, out[1] gates in blocks
+ * (X split per target), out[2] gates run as their own launch. No device call. */
+int yh_block_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[3]);
 /* Host-only plan of yh_apply_fused for a circuit on nbits qubits: out[0] HBM passes
  * of the tile kernel, out[1] gate groups, out[2] gates placed in tiles, out[3]
  * gates run as their own launch. No device call. */

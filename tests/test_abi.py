@@ -143,3 +143,20 @@ def test_tile_plan_covers_every_gate():
     passes, groups, tiled, single = [int(x) for x in out]
     assert tiled == len(els) and single == 0
     assert passes <= 60 and groups >= passes
+
+
+def test_block_plan_covers_every_gate():
+    """Host-only 5-qubit block planner on the 28-qubit depth-20 brickwork: every
+    gate lands in a block; far fewer blocks than gates. Wide gates run alone."""
+    from fixture_circuits import brickwork
+    els = brickwork(28, 20, 20)
+    ol = lower(els)
+    out = np.zeros(3, dtype=np.int64)
+    A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
+    blocks, inblk, single = [int(x) for x in out]
+    assert inblk == len(els) and single == 0
+    assert blocks <= 80
+    wide = [PositionedGate(Gate.Custom(np.eye(8), False), [0, 1, 2], [3, 4, 5], [True, True, False])]
+    ol = lower(els[:10] + wide)
+    A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
+    assert int(out[2]) == 1 and int(out[1]) == 10

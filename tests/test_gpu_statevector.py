@@ -297,3 +297,55 @@ def test_fused_matches_gate_by_gate_26q(gpu):
     a = run_gpu(gpu, n, psi, els, fused=0)
     b = run_gpu(gpu, n, psi, els, fused=4)
     assert rel_err(b, a) < TOL
+
+
+# ---------------- 5-qubit MFMA blocks (yh_apply_fused, max_fuse_qubits = 5) ----------------
+@pytest.mark.parametrize("n,seed", [(5, 0), (6, 1), (9, 2), (11, 3), (12, 4), (16, 5)])
+def test_block5_random_circuits_vs_oracle(gpu, n, seed):
+    """All dispatch classes; gates on more than 5 qubits run on their own. n < 11
+    leaves fewer than 64 columns per block (partial workgroup steps)."""
+    els = random_circuit(n, 200, 900 + seed, max_ctrls=3, custom_targets=(1, 2, 3, 4, 5, 6))
+    psi = seeded_state(n, 50 + seed)
+    print(f"[case] block5 random n={n} seed={seed}", flush=True)
+    got = run_gpu(gpu, n, psi, els, fused=5)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+@pytest.mark.parametrize("n,depth", [(14, 6), (20, 4)])
+def test_block5_brickwork_vs_oracle(gpu, n, depth):
+    els = brickwork(n, depth, 60 + n)
+    psi = seeded_state(n, 60 + n)
+    print(f"[case] block5 brickwork n={n} depth={depth}", flush=True)
+    got = run_gpu(gpu, n, psi, els, fused=5)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+def test_block5_qft_and_controlled_x(gpu):
+    n = 16
+    psi = seeded_state(n, 17)
+    els = qft_with_swaps(n) + [PositionedGate(Gate.X, [7], [3, 9], [True, False]),
+                               PositionedGate(Gate.X, [15], [0, 1, 2, 3, 4], [True] * 5)]
+    print("[case] block5 qft16 + controlled X", flush=True)
+    got = run_gpu(gpu, n, psi, els, fused=5)
+    assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+def test_block5_complex64_error(gpu):
+    n = 14
+    els = brickwork(n, 6, 5)
+    psi = seeded_state(n, 5)
+    print("[case] block5 complex64", flush=True)
+    got = run_gpu(gpu, n, psi, els, "c64", fused=5)
+    err = rel_err(got, O.apply(n, psi, lower(els)))
+    print(f"complex64 block5 relative error n={n}: {err:.3e}")
+    assert err < 2e-5
+
+
+def test_block5_matches_gate_by_gate_26q(gpu):
+    n = 26
+    els = brickwork(n, 4, 9)
+    psi = seeded_state(n, 9)
+    print("[case] block5 vs gate-by-gate 26q", flush=True)
+    a = run_gpu(gpu, n, psi, els, fused=0)
+    b = run_gpu(gpu, n, psi, els, fused=5)
+    assert rel_err(b, a) < TOL

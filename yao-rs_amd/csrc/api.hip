@@ -574,6 +574,10 @@ int yh_apply_fused(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops
   YH_GUARD(max_fuse_qubits >= 0 && max_fuse_qubits <= 5, YH_ERR_INVALID_ARG, "max_fuse_qubits must be in [0, 5]");
   YH_GUARD(circuit_nbits == reg->nbits, YH_ERR_NBITS_MISMATCH, "Register and circuit qubit count mismatch");
   YH_TRY(yh_validate(circuit_nbits, ops, nops));
+  if (max_fuse_qubits == 5) {  // 5-qubit blocks, one MFMA pass each
+    if (reg->dtype == YH_C128) return apply_blocks<double>(reg->w, reg->state.p, reg->nbits, ops, nops);
+    return apply_blocks<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
+  }
   if (reg->nbits < 12) {  // smaller than one tile: the per-gate path
     if (reg->dtype == YH_C128) return apply_gates<double>(reg->w, reg->state.p, reg->nbits, ops, nops);
     return apply_gates<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
@@ -586,6 +590,12 @@ int yh_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]) {
   YH_GUARD(out, YH_ERR_INVALID_ARG, "null argument");
   YH_TRY(yh_validate(nbits, ops, nops));
   return plan_stats(nbits, ops, nops, out);
+}
+
+int yh_block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]) {
+  YH_GUARD(out, YH_ERR_INVALID_ARG, "null argument");
+  YH_TRY(yh_validate(nbits, ops, nops));
+  return block_plan_stats(nbits, ops, nops, out);
 }
 
 int yh_instruct(yh_reg* reg, const yh_op* op) {

@@ -72,5 +72,8 @@ int apply_gates(Workspace& w, void* state, int nbits, const yh_op* ops, size_t n
 template <class R>
 int apply_tiled(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
 int plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]);
+template <class R>
+int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
+int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]);
 
 }  // namespace yh

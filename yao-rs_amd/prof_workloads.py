@@ -1,7 +1,9 @@
 """Named workloads for rocprofv3 runs (kernel traces and PMC counters).
 
 usage: python yao-rs_amd/prof_workloads.py WORKLOAD [nbits] [depth]
-  tiled      random brickwork through yh_apply_fused (one step)
+  tiled      random brickwork through yh_apply_fused, LDS tile passes (one step)
+  block      the same circuit as 5-qubit MFMA blocks (one step, complex128)
+  block64    the same circuit as 5-qubit MFMA blocks (one step, complex64)
   gbg        the same circuit gate by gate (one step)
   copy       one device-to-device copy of a 2^nbits complex128 state (the
              reference kernel for scaling FETCH_SIZE / WRITE_SIZE)
@@ -28,11 +30,11 @@ def main():
     import yaohip
     from yaohip.circuit import Circuit
     from fixture_circuits import brickwork, noisy_brickwork
-    if w in ("tiled", "gbg"):
-        reg = yaohip.ArrayReg.zero_state(n)
+    if w in ("tiled", "gbg", "block", "block64"):
+        reg = yaohip.ArrayReg.zero_state(n, "c64" if w == "block64" else "c128")
         circ = Circuit.qubits(n, brickwork(n, depth, 20))
         t0 = time.perf_counter()
-        yaohip.apply_inplace(circ, reg, 4 if w == "tiled" else 0)
+        yaohip.apply_inplace(circ, reg, {"tiled": 4, "gbg": 0, "block": 5, "block64": 5}[w])
         print(f"[prof] {w} {time.perf_counter() - t0:.3f} s norm {reg.norm():.15f}", flush=True)
     elif w == "copy":
         a = yaohip.ArrayReg.zero_state(n)
