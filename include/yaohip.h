@@ -147,6 +147,17 @@ int yh_block_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[3]
  * of the tile kernel, out[1] gate groups, out[2] gates placed in tiles, out[3]
  * gates run as their own launch. No device call. */
 int yh_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[4]);
+/* Circuit::num_params (circuit.rs:308-316): trainable parameters of the gate
+ * elements (Rx, Ry, Rz, Phase: 1; FSim: 2), in Circuit::parameters order. */
+int yh_circuit_num_params(const yh_op *ops, size_t nops, size_t *n);
+/* expect_grad (ad.rs:75-156): value = Re <psi|H|psi> with psi = circuit |reg>, and
+ * its gradient with respect to every trainable parameter (ngrad must equal
+ * yh_circuit_num_params), by one forward pass and an adjoint sweep on the device.
+ * `reg` is not modified. H is flattened as for yh_expect (MSB-first sites). A
+ * circuit with a noise channel returns YH_ERR_INVALID_ARG. */
+int yh_expect_grad(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops, size_t nterms,
+                   const double *coeffs, const int32_t *term_len, const int32_t *sites,
+                   const int32_t *opcodes, double *value, double *grad, size_t ngrad);
 /* dispatch_arrayreg_gate for one gate element (apply.rs:7). */
 int yh_instruct(yh_reg *reg, const yh_op *op);
 /* Host-side gate matrix, as Gate::matrix (gate.rs:63-68, 284-393); out holds

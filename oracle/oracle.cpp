@@ -827,6 +827,93 @@ yh_op custom_gate(const Mat& m, int d, bool diag, const yh_op& pos, std::vector<
   return g;
 }
 
+// ---------------- adjoint differentiation (src/ad.rs) ----------------
+int gate_num_params(const yh_op& op) {  // gate.rs:107-113
+  if (op.elem != YH_ELEM_GATE) return 0;
+  if (op.kind == YH_GATE_RX || op.kind == YH_GATE_RY || op.kind == YH_GATE_RZ || op.kind == YH_GATE_PHASE) return 1;
+  return op.kind == YH_GATE_FSIM ? 2 : 0;
+}
+
+Mat generator_matrix(const yh_op& op, int i) {  // gate.rs:153-193
+  C z = cx(0, 0);
+  switch (op.kind) {
+    case YH_GATE_RX: return m2(z, cx(0, -0.5), cx(0, -0.5), z);
+    case YH_GATE_RY: return m2(z, cx(-0.5, 0), cx(0.5, 0), z);
+    case YH_GATE_RZ: return m2(cx(0, -0.5), z, z, cx(0, 0.5));
+    case YH_GATE_PHASE: return m2(z, z, z, cx(0, 1));
+    default: {
+      Mat m(16, z);
+      if (i == 0) {
+        m[1 * 4 + 2] = cx(0, -1);
+        m[2 * 4 + 1] = cx(0, -1);
+      } else {
+        m[3 * 4 + 3] = cx(0, -1);
+      }
+      return m;
+    }
+  }
+}
+
+Mat conjugate_transpose(const Mat& m, int d) {
+  Mat o(m.size());
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c < d; ++c) o[r * d + c] = conj(m[c * d + r]);
+  return o;
+}
+
+// Gate::dagger (gate.rs:198-281), with the same element position
+yh_op dagger_gate(const yh_op& op, std::vector<double>& store) {
+  yh_op d = op;
+  switch (op.kind) {
+    case YH_GATE_H: case YH_GATE_X: case YH_GATE_Y: case YH_GATE_Z: case YH_GATE_SWAP:
+      return d;
+    case YH_GATE_S:
+      d.kind = YH_GATE_PHASE;
+      d.params[0] = -M_PI / 2;
+      return d;
+    case YH_GATE_T:
+      d.kind = YH_GATE_PHASE;
+      d.params[0] = -M_PI / 4;
+      return d;
+    case YH_GATE_RX: case YH_GATE_RY: case YH_GATE_RZ: case YH_GATE_PHASE:
+      d.params[0] = -op.params[0];
+      return d;
+    case YH_GATE_FSIM:
+      d.params[0] = -op.params[0];
+      d.params[1] = -op.params[1];
+      return d;
+    default: {  // SqrtX, SqrtY, SqrtW, ISWAP, Custom: conjugate transpose as Custom
+      std::vector<C> m;
+      int dim = gate_matrix(op, m);
+      bool diag = op.kind == YH_GATE_CUSTOM ? op.is_diag != 0 : false;
+      return custom_gate(conjugate_transpose(m, dim), dim, diag, op, store);
+    }
+  }
+}
+
+bool controls_match(usize basis, usize nbits, const yh_op& op) {  // ad.rs:19-27
+  for (int k = 0; k < op.nc; ++k) {
+    usize bit = nbits - 1 - usize(op.ctrls[k]);
+    if (((basis >> bit) & 1) != usize(op.ctrl_vals[k])) return false;
+  }
+  return true;
+}
+
+// apply_generator (ad.rs:29-64): the generator as a non-diagonal Custom gate at
+// the same position; amplitudes on inactive control branches become zero
+void apply_generator(Vec& state, usize nbits, const yh_op& op, const Mat& g) {
+  std::vector<double> store;
+  int d = 1 << op.nt;
+  yh_op temp = custom_gate(g, d, false, op, store);
+  if (op.nc == 0) {
+    dispatch(nbits, state.data(), temp);
+    return;
+  }
+  Vec transformed = state;
+  dispatch(nbits, transformed.data(), temp);
+  for (usize b = 0; b < state.size(); ++b) state[b] = controls_match(b, nbits, op) ? transformed[b] : cx(0, 0);
+}
+
 void dm_apply_gate(usize n, C* rho, const yh_op& op) {  // :161-178
   dispatch(2 * n, rho, op);
   std::vector<C> m;
@@ -1060,6 +1147,57 @@ int orc_expect(int nbits, const double* state, size_t nterms, const double* coef
   for (auto& term : t) acc += term.coeff * expect_opstring_pure(st, term);
   out[0] = acc.re;
   out[1] = acc.im;
+  return 0;
+}
+
+// expect_grad (ad.rs:75-156). Returns 1 for a circuit with a channel (the
+// reference panics), 2 for a wrong gradient length.
+int orc_expect_grad(int nbits, const double* psi0, const void* opsv, size_t nops, size_t nterms, const double* coeffs,
+                    const int32_t* term_len, const int32_t* sites, const int32_t* opcodes, double* value,
+                    double* grad, size_t ngrad) {
+  const yh_op* ops = static_cast<const yh_op*>(opsv);
+  for (size_t e = 0; e < nops; ++e)
+    if (ops[e].elem == YH_ELEM_CHANNEL) return 1;
+  usize len = usize(1) << nbits;
+  Vec psi(asC(psi0), asC(psi0) + len);
+  for (size_t e = 0; e < nops; ++e)  // apply_inplace
+    if (ops[e].elem == YH_ELEM_GATE) dispatch(nbits, psi.data(), ops[e]);
+  std::vector<Term> terms = read_terms(nterms, coeffs, term_len, sites, opcodes);
+  C v = cx(0, 0);
+  for (auto& t : terms) v += t.coeff * expect_opstring_pure(psi, t);
+  *value = v.re;
+  size_t n_params = 0;
+  for (size_t e = 0; e < nops; ++e) n_params += size_t(gate_num_params(ops[e]));
+  if (ngrad != n_params) return 2;
+  if (n_params == 0) return 0;
+  Vec lambda(len, cx(0, 0));
+  for (auto& t : terms) {
+    Vec tmp = psi;
+    for (auto& so : t.ops) {
+      Mat m = op_matrix(so.second);
+      instruct_1q(tmp.data(), len, so.first, m[0], m[1], m[2], m[3]);
+    }
+    for (usize i = 0; i < len; ++i) lambda[i] += t.coeff * tmp[i];
+  }
+  size_t slot = n_params;
+  Vec phi(len);
+  std::vector<double> store;
+  for (size_t e = nops; e-- > 0;) {
+    const yh_op& op = ops[e];
+    if (op.elem != YH_ELEM_GATE) continue;
+    int np = gate_num_params(op);
+    for (int i = np - 1; i >= 0; --i) {
+      --slot;
+      phi = psi;
+      apply_generator(phi, nbits, op, generator_matrix(op, i));
+      C acc = cx(0, 0);
+      for (usize k = 0; k < len; ++k) acc += conj(lambda[k]) * phi[k];
+      grad[slot] = 2.0 * acc.re;
+    }
+    yh_op d = dagger_gate(op, store);
+    dispatch(nbits, psi.data(), d);
+    dispatch(nbits, lambda.data(), d);
+  }
   return 0;
 }
 

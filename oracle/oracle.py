@@ -55,6 +55,8 @@ def lib():
             "orc_dm_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
             "orc_dm_partial_tr": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d),
                                                  P(ctypes.c_int)]),
+            "orc_expect_grad": (ctypes.c_int, [ctypes.c_int, P(d), vp, ctypes.c_size_t, ctypes.c_size_t, P(d), P(i32),
+                                               P(i32), P(i32), P(d), P(d), ctypes.c_size_t]),
             "orc_mlp_loss_grad": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(d),
                                                  P(ctypes.c_float), P(ctypes.c_float), ctypes.c_int, ctypes.c_int,
                                                  P(d), P(d), P(d)]),
@@ -274,3 +276,13 @@ def mlp_loss_grad(in_dim, width, hidden, params, x, y, quant=True, trig=False):
     _ck(lib().orc_mlp_loss_grad(in_dim, int(bool(trig)), width, hidden, _d(params), x.ctypes.data_as(fp),
                                 y.ctypes.data_as(fp), n, int(bool(quant)), _d(out), _d(loss), _d(grads)))
     return out, float(loss[0]), grads
+
+
+def expect_grad(nbits, psi0, oplist, poly, nparams):
+    """expect_grad (ad.rs:75-156) on the CPU: returns (value, gradient)."""
+    T, coeffs, tl, sites, ops = poly.flat()
+    grad = np.zeros(max(1, nparams))
+    val = np.zeros(1)
+    _ck(lib().orc_expect_grad(nbits, _d(_il(psi0)), oplist.arr, oplist.n, T, _d(coeffs), _i(tl), _i(sites),
+                              _i(ops), _d(val), _d(grad), nparams))
+    return float(val[0]), grad[:nparams]

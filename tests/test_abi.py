@@ -160,3 +160,12 @@ def test_block_plan_covers_every_gate():
     ol = lower(els[:10] + wide)
     A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
     assert int(out[2]) == 1 and int(out[1]) == 10
+
+
+def test_circuit_num_params_matches_python_mirror():
+    from test_ad_oracle import random_param_circuit
+    els = random_param_circuit(5, 60, 3)
+    c = Circuit.qubits(5, els)
+    n = ctypes.c_size_t()
+    A.check(A.lib().yh_circuit_num_params(c.ops.arr, c.ops.n, ctypes.byref(n)))
+    assert n.value == c.num_params() == len(c.parameters())

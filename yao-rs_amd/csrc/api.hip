@@ -666,6 +666,238 @@ int yh_expect(const yh_reg* reg, size_t nterms, const double* coeffs, const int3
   return expect_impl(r->w, r->nbits, r->state.p, r->dtype, false, nterms, coeffs, term_len, sites, ops, out);
 }
 
+extern "C++" {
+// ------------------------- adjoint differentiation --------------------------
+// Gate::num_params (gate.rs:107-113).
+static int gate_num_params(const yh_op& op) {
+  if (op.elem != YH_ELEM_GATE) return 0;
+  switch (op.kind) {
+    case YH_GATE_RX:
+    case YH_GATE_RY:
+    case YH_GATE_RZ:
+    case YH_GATE_PHASE:
+      return 1;
+    case YH_GATE_FSIM:
+      return 2;
+    default:
+      return 0;
+  }
+}
+
+// Gate::generator_matrix (gate.rs:153-193): G = (dU/dtheta_i) U^-1, angle independent.
+static Mat generator_matrix(const yh_op& op, int i) {
+  const cd z{0, 0};
+  switch (op.kind) {
+    case YH_GATE_RX: return {z, cd{0, -0.5}, cd{0, -0.5}, z};
+    case YH_GATE_RY: return {z, cd{-0.5, 0}, cd{0.5, 0}, z};
+    case YH_GATE_RZ: return {cd{0, -0.5}, z, z, cd{0, 0.5}};
+    case YH_GATE_PHASE: return {z, z, z, cd{0, 1}};
+    default: {  // FSim
+      Mat m(16, z);
+      if (i == 0) {
+        m[1 * 4 + 2] = cd{0, -1};
+        m[2 * 4 + 1] = cd{0, -1};
+      } else {
+        m[3 * 4 + 3] = cd{0, -1};
+      }
+      return m;
+    }
+  }
+}
+
+// Gate::dagger (gate.rs:198-281) as an element; `store` keeps a Custom matrix alive.
+static int dagger_op(const yh_op& op, yh_op& d, std::vector<double>& store) {
+  d = op;
+  auto custom_from = [&](const Mat& m, int dim, int is_diag) {
+    store.assign(size_t(dim) * dim * 2, 0.0);
+    for (int r = 0; r < dim; ++r)
+      for (int c = 0; c < dim; ++c) {  // (m^T)* : out[r][c] = conj(m[c][r])
+        store[2 * (size_t(r) * dim + c)] = m[size_t(c) * dim + r].re;
+        store[2 * (size_t(r) * dim + c) + 1] = -m[size_t(c) * dim + r].im;
+      }
+    d.kind = YH_GATE_CUSTOM;
+    d.matrix = store.data();
+    d.dim = dim;
+    d.is_diag = is_diag;
+  };
+  switch (op.kind) {
+    case YH_GATE_H:
+    case YH_GATE_X:
+    case YH_GATE_Y:
+    case YH_GATE_Z:
+    case YH_GATE_SWAP:
+      return YH_OK;
+    case YH_GATE_S:
+      d.kind = YH_GATE_PHASE;
+      d.params[0] = -1.5707963267948966;  // -pi/2
+      return YH_OK;
+    case YH_GATE_T:
+      d.kind = YH_GATE_PHASE;
+      d.params[0] = -0.7853981633974483;  // -pi/4
+      return YH_OK;
+    case YH_GATE_RX:
+    case YH_GATE_RY:
+    case YH_GATE_RZ:
+    case YH_GATE_PHASE:
+      d.params[0] = -op.params[0];
+      return YH_OK;
+    case YH_GATE_FSIM:
+      d.params[0] = -op.params[0];
+      d.params[1] = -op.params[1];
+      return YH_OK;
+    case YH_GATE_CUSTOM: {
+      Mat m;
+      const int dim = gate_matrix(op, m);
+      custom_from(m, dim, op.is_diag);
+      return YH_OK;
+    }
+    default: {  // SqrtX, SqrtY, SqrtW, ISWAP: conjugate transpose as a Custom gate
+      Mat m;
+      const int dim = gate_matrix(op, m);
+      custom_from(m, dim, 0);
+      return YH_OK;
+    }
+  }
+}
+
+struct TermArgs {
+  size_t nterms;
+  const double* coeffs;
+  const int32_t* term_len;
+  const int32_t* sites;
+  const int32_t* opcodes;
+};
+
+template <class R>
+static int expect_grad_impl(yh_reg* reg, int nbits, const yh_op* ops, size_t nops, const TermArgs& ta,
+                            const std::vector<TermDev>& td, const std::vector<cd>& cf, size_t nparams, double* value,
+                            double* grad) {
+  using V = typename Cplx<R>::V;
+  Workspace& w = reg->w;
+  const uint64_t n = uint64_t(1) << nbits;
+  DevBuf psi, lam, gout, gens, coeffs;
+  YH_TRY(psi.ensure(reg->bytes()));
+  YH_TRY(lam.ensure(reg->bytes()));
+  YH_HIP(hipMemcpyAsync(psi.p, reg->state.p, reg->bytes(), hipMemcpyDeviceToDevice, w.stream));
+  YH_TRY(apply_gates<R>(w, psi.p, nbits, ops, nops));
+  // value = Re expect_arrayreg(psi, H) (ad.rs:92)
+  double ev[2];
+  YH_TRY(expect_impl(w, nbits, psi.p, reg->dtype, false, ta.nterms, ta.coeffs, ta.term_len, ta.sites, ta.opcodes, ev));
+  *value = ev[0];
+  if (nparams == 0) return YH_OK;
+  V* P = static_cast<V*>(psi.p);
+  V* Lm = static_cast<V*>(lam.p);
+  // lambda = sum_t c_t P_t psi (ad.rs:101-116)
+  if (td.empty()) {
+    YH_HIP(hipMemsetAsync(lam.p, 0, reg->bytes(), w.stream));
+  } else {
+    std::vector<double> c2(2 * cf.size());
+    for (size_t k = 0; k < cf.size(); ++k) {
+      c2[2 * k] = cf[k].re;
+      c2[2 * k + 1] = cf[k].im;
+    }
+    YH_TRY(w.upload(td.data(), td.size() * sizeof(TermDev), w.terms));
+    YH_TRY(w.upload(c2.data(), c2.size() * sizeof(double), coeffs));
+    for (size_t t0 = 0; t0 < td.size(); t0 += kMaxTermsPerPass) {
+      const int nt = int(std::min<size_t>(kMaxTermsPerPass, td.size() - t0));
+      YH_TRY(launch_pauli_accum<R>(P, Lm, n, static_cast<const TermDev*>(w.terms.p) + t0,
+                                   static_cast<const double*>(coeffs.p) + 2 * t0, nt, t0 == 0, w.stream));
+    }
+  }
+  // all generator matrices, in parameter order, uploaded once
+  std::vector<double2> gm;
+  std::vector<size_t> goff(nparams);
+  {
+    size_t slot = 0;
+    for (size_t e = 0; e < nops; ++e)
+      for (int i = 0; i < gate_num_params(ops[e]); ++i) {
+        const Mat g = generator_matrix(ops[e], i);
+        goff[slot++] = gm.size();
+        for (const cd& v : g) gm.push_back(double2{v.re, v.im});
+      }
+  }
+  YH_TRY(w.upload(gm.data(), gm.size() * sizeof(double2), gens));
+  YH_TRY(gout.ensure(2 * nparams * sizeof(double)));
+  // reverse sweep (ad.rs:118-152)
+  size_t slot = nparams;
+  std::vector<double> store;
+  for (size_t e = nops; e-- > 0;) {
+    const yh_op& op = ops[e];
+    if (op.elem != YH_ELEM_GATE) continue;
+    const int np = gate_num_params(op);
+    if (np > 0) {
+      auto bit_of = [nbits](int loc) { return nbits - 1 - loc; };
+      GroupArgs ga{};
+      std::vector<int> pos;
+      for (int k = 0; k < op.nt; ++k) pos.push_back(bit_of(op.targets[k]));
+      for (int k = 0; k < op.nc; ++k) {
+        pos.push_back(bit_of(op.ctrls[k]));
+        if (op.ctrl_vals[k]) ga.ctrl_set |= uint64_t(1) << bit_of(op.ctrls[k]);
+      }
+      for (int j = 0; j < (1 << op.nt); ++j) {
+        uint64_t o = 0;
+        for (int k = 0; k < op.nt; ++k)
+          if ((j >> k) & 1) o |= uint64_t(1) << bit_of(op.targets[k]);
+        ga.offs[j] = o;
+      }
+      set_locked(ga, nbits, pos);
+      for (int i = np - 1; i >= 0; --i) {
+        --slot;
+        YH_TRY(launch_gen_inner<R>(P, Lm, ga, op.nt, static_cast<const double2*>(gens.p) + goff[slot], w.partial(),
+                                   static_cast<double*>(gout.p) + 2 * slot, w.stream));
+      }
+    }
+    yh_op d;
+    YH_TRY(dagger_op(op, d, store));
+    YH_TRY(apply_gates<R>(w, psi.p, nbits, &d, 1));
+    YH_TRY(apply_gates<R>(w, lam.p, nbits, &d, 1));
+  }
+  std::vector<double> g2(2 * nparams);
+  if (nparams) YH_TRY(w.fetch(g2.data(), gout.p, g2.size() * sizeof(double)));
+  YH_TRY(w.sync());
+  for (size_t k = 0; k < nparams; ++k) grad[k] = 2.0 * g2[2 * k];
+  return YH_OK;
+}
+
+}  // extern "C++"
+
+int yh_circuit_num_params(const yh_op* ops, size_t nops, size_t* n) {
+  YH_GUARD(n && (nops == 0 || ops), YH_ERR_INVALID_ARG, "null argument");
+  size_t c = 0;
+  for (size_t e = 0; e < nops; ++e) c += size_t(gate_num_params(ops[e]));
+  *n = c;
+  return YH_OK;
+}
+
+int yh_expect_grad(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops, size_t nterms,
+                   const double* coeffs, const int32_t* term_len, const int32_t* sites, const int32_t* opcodes,
+                   double* value, double* grad, size_t ngrad) {
+  YH_GUARD(reg && value && (nops == 0 || ops), YH_ERR_INVALID_ARG, "null argument");
+  for (size_t e = 0; e < nops; ++e)
+    YH_GUARD(ops[e].elem != YH_ELEM_CHANNEL, YH_ERR_INVALID_ARG,
+             "expect_grad: circuit contains a noise channel; AD is unsupported for channels");
+  YH_GUARD(circuit_nbits == reg->nbits, YH_ERR_NBITS_MISMATCH, "Register and circuit qubit count mismatch");
+  YH_TRY(yh_validate(circuit_nbits, ops, nops));
+  size_t np = 0;
+  YH_TRY(yh_circuit_num_params(ops, nops, &np));
+  YH_GUARD(ngrad == np, YH_ERR_INVALID_ARG, "gradient length does not match the circuit's parameter count");
+  YH_GUARD(np == 0 || grad, YH_ERR_INVALID_ARG, "null gradient buffer");
+  size_t total = 0;
+  for (size_t k = 0; k < nterms; ++k) {
+    YH_GUARD(term_len && term_len[k] >= 0, YH_ERR_INVALID_ARG, "bad term length");
+    total += size_t(term_len[k]);
+  }
+  YH_GUARD(nterms == 0 || coeffs, YH_ERR_INVALID_ARG, "null coefficients");
+  YH_GUARD(total == 0 || (sites && opcodes), YH_ERR_INVALID_ARG, "null sites/ops");
+  std::vector<Term> terms = read_terms(nterms, coeffs, term_len, sites, opcodes);
+  std::vector<TermDev> td;
+  std::vector<cd> cf;
+  YH_TRY(lower_terms(reg->nbits, terms, false, td, cf));
+  const TermArgs ta{nterms, coeffs, term_len, sites, opcodes};
+  if (reg->dtype == YH_C128) return expect_grad_impl<double>(reg, reg->nbits, ops, nops, ta, td, cf, np, value, grad);
+  return expect_grad_impl<float>(reg, reg->nbits, ops, nops, ta, td, cf, np, value, grad);
+}
+
 static int measure_locs(int nbits, const int32_t* locs, int nlocs, std::vector<int32_t>& locbits) {
   YH_GUARD(nlocs >= 0 && nlocs <= nbits && nlocs <= 30, YH_ERR_MEASURE_LOC, "too many measured locations");
   YH_GUARD(nlocs == 0 || locs, YH_ERR_INVALID_ARG, "null locs");

@@ -57,6 +57,14 @@ struct TermDev {
   double fac[kMaxSites][2][2];  // [site][v] = (re, im)
 };
 
+// ---- ad_kernels.hip ----
+template <class R>
+int launch_pauli_accum(const typename Cplx<R>::V* psi, typename Cplx<R>::V* lam, uint64_t n, const TermDev* terms,
+                       const double* coeffs, int nt, bool first, hipStream_t st);
+template <class R>
+int launch_gen_inner(const typename Cplx<R>::V* psi, const typename Cplx<R>::V* lam, const GroupArgs& ga, int nt,
+                     const double2* G, double* partial, double* out, hipStream_t st);
+
 template <class R>
 int launch_norm2(const typename Cplx<R>::V* s, uint64_t n, double* partial, double* out,
                  hipStream_t st);

@@ -129,6 +129,9 @@ SIGNATURES = [
     ("yh_apply", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
     ("yh_apply_fused", c_int, [c_void_p, c_int, P(YhOp), c_size_t, c_int]),
     ("yh_instruct", c_int, [c_void_p, P(YhOp)]),
+    ("yh_circuit_num_params", c_int, [P(YhOp), c_size_t, P(c_size_t)]),
+    ("yh_expect_grad", c_int, [c_void_p, c_int, P(YhOp), c_size_t, c_size_t, P(c_double), P(ctypes.c_int32),
+                               P(ctypes.c_int32), P(ctypes.c_int32), P(c_double), P(c_double), c_size_t]),
     ("yh_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
     ("yh_block_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
     ("yh_gate_matrix", c_int, [P(YhOp), P(c_double), P(c_int)]),

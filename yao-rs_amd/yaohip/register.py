@@ -128,6 +128,20 @@ def expect_arrayreg(reg, op: OperatorPolynomial):
     return complex(out[0], out[1])
 
 
+def expect_grad(observable: OperatorPolynomial, circuit, psi0):
+    """expect_grad (ad.rs:75-156): (Re <psi|H|psi>, gradient over circuit.parameters())
+    with psi = circuit |psi0>; psi0 is left unchanged."""
+    T, coeffs, tl, sites, ops = observable.flat()
+    ol = circuit.ops
+    n = ctypes.c_size_t()
+    A.check(A.lib().yh_circuit_num_params(ol.arr, ol.n, ctypes.byref(n)))
+    grad = np.zeros(max(1, n.value))
+    val = ctypes.c_double()
+    A.check(A.lib().yh_expect_grad(psi0._h, circuit.nbits, ol.arr, ol.n, T, A.dptr(coeffs), A.i32ptr(tl),
+                                   A.i32ptr(sites), A.i32ptr(ops), ctypes.byref(val), A.dptr(grad), n.value))
+    return val.value, grad[:n.value]
+
+
 def probs(state, locs=None):
     """probs (measure.rs:82-87) for an ArrayReg or DensityMatrix."""
     n = state.nqubits()
