@@ -202,6 +202,15 @@ int yh_dm_apply(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
 int yh_dm_apply_fused(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
 int yh_dm_trace(const yh_dm *dm, double out[2]);                   /* trace :63-67      */
 int yh_dm_purity(const yh_dm *dm, double *out);                    /* purity :69-78     */
+/* partial_tr (density_matrix.rs:80-111): the reduced density matrix over the
+ * kept qubits, as a new handle. `traced` are BIT positions (LSB-first, like the
+ * reference's loc here); kept bit idx of the reduced index is the idx-th kept bit.
+ * Entries are summed in the reference's order (bit-identical). */
+int yh_dm_partial_tr(const yh_dm *dm, const int32_t *traced, int ntraced, yh_dm **out);
+/* von_neumann_entropy (density_matrix.rs:113-122), nbits <= 10: the matrix is
+ * downloaded and its Hermitian eigenvalues computed on the host (Householder
+ * tridiagonalisation + implicit QL), as the reference does with faer. */
+int yh_dm_entropy(const yh_dm *dm, double *out);
 /* expect_dm (expect.rs:38-72): Pauli sites are LSB-first bit positions. */
 int yh_dm_expect(const yh_dm *dm, size_t nterms, const double *coeffs,
                  const int32_t *term_len, const int32_t *sites, const int32_t *ops,

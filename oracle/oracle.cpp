@@ -1396,6 +1396,59 @@ int orc_dm_probs(int nbits, const double* rho, const int32_t* locs, int nlocs, d
 }
 
 // partial_tr (density_matrix.rs:80-112), traced locs are LSB-first bit positions.
+// von_neumann_entropy (density_matrix.rs:113-122). Eigenvalues by cyclic Jacobi
+// rotations on the real symmetric embedding [[A, -B], [B, A]] of rho = A + iB
+// (each eigenvalue appears twice), then the reference's clamp and filter.
+int orc_dm_entropy(int nbits, const double* rho, double* out) {
+  const usize d = usize(1) << nbits, n = 2 * d;
+  const C* r = asC(rho);
+  std::vector<double> a(n * n);
+  for (usize i = 0; i < d; ++i)
+    for (usize j = 0; j < d; ++j) {
+      const C h = r[i * d + j];
+      a[i * n + j] = h.re;
+      a[(i + d) * n + j + d] = h.re;
+      a[i * n + j + d] = -h.im;
+      a[(i + d) * n + j] = h.im;
+    }
+  // a sweep rotates every pair whose off-diagonal entry is not negligible against
+  // its two diagonal entries; the iteration ends after a sweep with no rotation
+  for (int sweep = 0; sweep < 60; ++sweep) {
+    int rotated = 0;
+    for (usize p = 0; p + 1 < n; ++p)
+      for (usize q = p + 1; q < n; ++q) {
+        const double apq = a[p * n + q];
+        const double scale = std::fabs(a[p * n + p]) + std::fabs(a[q * n + q]);
+        if (std::fabs(apq) <= 1e-17 * scale || std::fabs(apq) < 1e-300) continue;
+        ++rotated;
+        const double theta = (a[q * n + q] - a[p * n + p]) / (2.0 * apq);
+        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+        for (usize k = 0; k < n; ++k) {  // columns p, q
+          const double kp = a[k * n + p], kq = a[k * n + q];
+          a[k * n + p] = c * kp - s * kq;
+          a[k * n + q] = s * kp + c * kq;
+        }
+        for (usize k = 0; k < n; ++k) {  // rows p, q
+          const double pk = a[p * n + k], qk = a[q * n + k];
+          a[p * n + k] = c * pk - s * qk;
+          a[q * n + k] = s * pk + c * qk;
+        }
+      }
+    if (rotated == 0) break;
+  }
+  std::vector<double> ev(n);
+  for (usize i = 0; i < n; ++i) ev[i] = a[i * n + i];
+  std::sort(ev.begin(), ev.end());
+  double acc = 0.0;
+  for (usize k = 0; k < d; ++k) {
+    double v = std::max(ev[2 * k], 0.0);
+    if (v > 1e-15) acc += v * std::log(v);
+  }
+  *out = -acc;
+  return 0;
+}
+
 int orc_dm_partial_tr(int nbits, const double* rho, const int32_t* traced, int ntr, double* out,
                       int* kept_nbits) {
   std::vector<usize> tl(traced, traced + ntr), kept;

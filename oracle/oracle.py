@@ -55,6 +55,7 @@ def lib():
             "orc_dm_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
             "orc_dm_partial_tr": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d),
                                                  P(ctypes.c_int)]),
+            "orc_dm_entropy": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
             "orc_expect_grad": (ctypes.c_int, [ctypes.c_int, P(d), vp, ctypes.c_size_t, ctypes.c_size_t, P(d), P(i32),
                                                P(i32), P(i32), P(d), P(d), ctypes.c_size_t]),
             "orc_mlp_loss_grad": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P(d),
@@ -286,3 +287,10 @@ def expect_grad(nbits, psi0, oplist, poly, nparams):
     _ck(lib().orc_expect_grad(nbits, _d(_il(psi0)), oplist.arr, oplist.n, T, _d(coeffs), _i(tl), _i(sites),
                               _i(ops), _d(val), _d(grad), nparams))
     return float(val[0]), grad[:nparams]
+
+
+def dm_entropy(nbits, rho):
+    """von_neumann_entropy (density_matrix.rs:113-122) of a 2^n x 2^n matrix."""
+    out = np.zeros(1)
+    _ck(lib().orc_dm_entropy(nbits, _d(_il(rho)), _d(out)))
+    return float(out[0])

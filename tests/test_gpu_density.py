@@ -130,3 +130,65 @@ def test_dm_large_trace_and_purity_14q(gpu):
     assert abs(dm.trace() - 1) < 1e-12
     p = dm.purity()
     assert 0.9 < p < 1.0
+
+
+def reference_noisy_circuit(nq):
+    """build_noisy_circuit of the reference's validation suite: H on all, CNOT
+    chain, Depolarizing(n=1, p=0.01) on all, Rz(0.3) on all, AmplitudeDamping(0.05)."""
+    els = [put([q], Gate.H) for q in range(nq)]
+    els += [control([q], [q + 1], Gate.X) for q in range(nq - 1)]
+    els += [channel([q], NoiseChannel.Depolarizing(1, 0.01)) for q in range(nq)]
+    els += [put([q], Gate.Rz(0.3)) for q in range(nq)]
+    els += [channel([q], NoiseChannel.AmplitudeDamping(0.05, 0.0)) for q in range(nq)]
+    return els
+
+
+@pytest.mark.parametrize("nq", [4, 6, 8, 10])
+def test_partial_tr_and_entropy_reference_circuit(gpu, nq):
+    els = reference_noisy_circuit(nq)
+    d, dk = 1 << nq, 1 << (nq - 1)
+    dm = gpu.DensityMatrix.zero_state(nq)
+    dm.apply(Circuit.qubits(nq, els))
+    rho = dm.state_data().reshape(d, d)
+    red = dm.partial_tr([nq - 1])
+    got = red.state_data().reshape(dk, dk)
+    # bit-identical to the oracle's partial trace of the same matrix (same summation order)
+    assert np.array_equal(got, O.dm_partial_tr(nq, rho, [nq - 1]))
+    # and close to the oracle's evolution
+    z = np.zeros((d, d), dtype=complex)
+    z[0, 0] = 1
+    want_rho = O.dm_apply(nq, z, lower(els)).reshape(d, d)
+    assert np.max(np.abs(got - O.dm_partial_tr(nq, want_rho, [nq - 1]))) < 1e-12
+    s = red.von_neumann_entropy()
+    if nq <= 8:
+        want_s = O.dm_entropy(nq - 1, got)
+    else:  # the oracle's Jacobi sweep is slow at 512 x 512; numpy's eigvalsh pins it
+        ev = np.maximum(np.linalg.eigvalsh(got), 0.0)
+        ev = ev[ev > 1e-15]
+        want_s = float(-np.sum(ev * np.log(ev)))
+    print(f"nq={nq}: reduced entropy {s:.12f} oracle {want_s:.12f}")
+    assert abs(s - want_s) < 1e-10
+
+
+def test_partial_tr_edge_cases(gpu):
+    import yaohip
+    n = 3
+    rng = np.random.default_rng(3)
+    a = rng.standard_normal((8, 8)) + 1j * rng.standard_normal((8, 8))
+    rho = a @ a.conj().T
+    rho /= np.trace(rho).real
+    dm = gpu.DensityMatrix.from_matrix(n, rho)
+    # every qubit traced: the 1 x 1 trace; no qubit traced: the matrix itself;
+    # traced locations at or beyond nbits trace nothing
+    full = dm.partial_tr([0, 1, 2])
+    assert full.nbits() == 0 and abs(full.state_data()[0] - np.trace(rho)) < 1e-14
+    assert full.von_neumann_entropy() == 0.0
+    assert np.array_equal(dm.partial_tr([]).state_data().reshape(8, 8), rho)
+    assert np.array_equal(dm.partial_tr([5]).state_data().reshape(8, 8), rho)
+    assert np.array_equal(dm.partial_tr([2, 0, 2]).state_data().reshape(2, 2), O.dm_partial_tr(n, rho, [2, 0]))
+    with pytest.raises(yaohip.YhError):
+        dm.partial_tr([-1])
+    # entropy of the mixed 3-qubit matrix against numpy's eigvalsh
+    ev = np.maximum(np.linalg.eigvalsh(rho), 0.0)
+    ev = ev[ev > 1e-15]
+    assert abs(dm.von_neumann_entropy() - float(-np.sum(ev * np.log(ev)))) < 1e-10

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -1101,6 +1102,55 @@ int yh_dm_upload(yh_dm* dm, const double* rho, size_t n) {
   YH_GUARD(dm && rho, YH_ERR_INVALID_ARG, "null argument");
   YH_GUARD(n == size_t(dm->dim()) * dm->dim(), YH_ERR_INVALID_ARG, "length must be 4^nbits");
   YH_HIP(hipMemcpy(dm->rho.p, rho, dm->bytes(), hipMemcpyHostToDevice));
+  return YH_OK;
+}
+
+int yh_dm_partial_tr(const yh_dm* cdm, const int32_t* traced, int ntraced, yh_dm** out) {
+  YH_GUARD(cdm && out && (ntraced == 0 || traced) && ntraced >= 0, YH_ERR_INVALID_ARG, "null argument");
+  yh_dm* dm = const_cast<yh_dm*>(cdm);
+  // traced locs are bit positions; locs at or beyond nbits trace nothing
+  // (density_matrix.rs:80-84 filters 0..nbits)
+  PtrArgs a{};
+  for (int l = 0; l < dm->nbits; ++l) {
+    bool t = false;
+    for (int k = 0; k < ntraced; ++k) {
+      YH_GUARD(traced[k] >= 0 && traced[k] < 64, YH_ERR_LOC_OUT_OF_RANGE, "traced location out of range");
+      if (traced[k] == l) t = true;
+    }
+    if (t)
+      a.tr[a.ntr++] = l;
+    else
+      a.kept[a.nkept++] = l;
+  }
+  std::unique_ptr<yh_dm> d(new yh_dm());
+  d->nbits = a.nkept;
+  YH_TRY(d->w.init());
+  YH_TRY(d->rho.ensure(d->bytes()));
+  YH_HIP(hipStreamSynchronize(dm->w.stream));
+  YH_TRY(launch_partial_tr(static_cast<const double2*>(dm->rho.p), dm->dim(), a, static_cast<double2*>(d->rho.p),
+                           d->w.stream));
+  YH_TRY(d->w.sync());
+  *out = d.release();
+  return YH_OK;
+}
+
+int yh_dm_entropy(const yh_dm* dm, double* out) {
+  YH_GUARD(dm && out, YH_ERR_INVALID_ARG, "null argument");
+  YH_GUARD(dm->nbits <= 10, YH_ERR_UNSUPPORTED, "von Neumann entropy is computed for at most 10 qubits");
+  const int d = int(dm->dim());
+  std::vector<double> buf(size_t(2) * d * d);
+  YH_TRY(yh_dm_download(dm, buf.data(), size_t(d) * d));
+  Mat m(size_t(d) * d);
+  for (size_t k = 0; k < m.size(); ++k) m[k] = cd{buf[2 * k], buf[2 * k + 1]};
+  // von_neumann_entropy (density_matrix.rs:113-122): eigenvalues clamped at 0,
+  // those above 1e-15 contribute -v ln v
+  std::vector<double> ev = hermitian_eigenvalues(m, d);
+  double s = 0.0;
+  for (double v : ev) {
+    v = std::max(v, 0.0);
+    if (v > 1e-15) s += v * std::log(v);
+  }
+  *out = -s;
   return YH_OK;
 }
 

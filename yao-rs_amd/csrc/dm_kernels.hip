@@ -265,7 +265,42 @@ __global__ __launch_bounds__(kBlock) void k_dm_outer(const double2* __restrict__
   rho[e] = cmul(a, b);
 }
 
+// partial_tr (density_matrix.rs:80-111): out[rr][rc] = sum over the traced bits t
+// of rho[row(rr, t)][col(rc, t)]. Traced and kept positions are bit positions
+// (the reference's loc), kept bit idx of the reduced index <- kept[idx]. One thread
+// per output entry sums in increasing t, which is the reference's row order, so
+// the sums are bit-identical to its loop.
+__global__ __launch_bounds__(kBlock) void k_partial_tr(const double2* __restrict__ rho, uint64_t dim,
+                                                       PtrArgs a, double2* __restrict__ out) {
+  const uint64_t dr = uint64_t(1) << a.nkept;
+  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (e >= dr * dr) return;
+  const uint64_t rr = e / dr, rc = e % dr;
+  uint64_t rb = 0, cb = 0;
+  for (int k = 0; k < a.nkept; ++k) {
+    rb |= ((rr >> k) & 1) << a.kept[k];
+    cb |= ((rc >> k) & 1) << a.kept[k];
+  }
+  const uint64_t nt = uint64_t(1) << a.ntr;
+  double sr = 0.0, si = 0.0;
+  for (uint64_t t = 0; t < nt; ++t) {
+    uint64_t tb = 0;
+    for (int k = 0; k < a.ntr; ++k) tb |= ((t >> k) & 1) << a.tr[k];
+    const double2 v = rho[(rb | tb) * dim + (cb | tb)];
+    sr += v.x;
+    si += v.y;
+  }
+  out[e] = double2{sr, si};
+}
+
 }  // namespace
+
+int launch_partial_tr(const double2* rho, uint64_t dim, const PtrArgs& a, double2* out, hipStream_t st) {
+  const uint64_t n = (uint64_t(1) << a.nkept) << a.nkept;
+  hipLaunchKernelGGL(k_partial_tr, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, rho, dim, a, out);
+  YH_HIP(hipGetLastError());
+  return YH_OK;
+}
 
 int launch_dm_block(double2* rho, const DmArgs& a, int K, const double2* mats, hipStream_t st) {
   dim3 grid(unsigned((a.ngroups + kBlock - 1) / kBlock)), block(kBlock);

@@ -352,4 +352,109 @@ void op_row_table(int op, int src[2], cd fac[2]) {
   }
 }
 
+// Hermitian eigenvalues through the real symmetric embedding S = [[A, -B], [B, A]]
+// of H = A + iB (every eigenvalue of H appears twice in S): Householder reduction
+// to tridiagonal form, then implicit-shift QL iterations (the textbook tred2/tqli
+// pair, eigenvalues only), then every second sorted value.
+std::vector<double> hermitian_eigenvalues(const Mat& m, int d) {
+  const int n = 2 * d;
+  std::vector<double> a(size_t(n) * n);
+  auto A = [&](int r, int c) -> double& { return a[size_t(r) * n + c]; };
+  for (int r = 0; r < d; ++r)
+    for (int c = 0; c < d; ++c) {
+      const cd h = m[size_t(r) * d + c];
+      A(r, c) = h.re;
+      A(r + d, c + d) = h.re;
+      A(r, c + d) = -h.im;
+      A(r + d, c) = h.im;
+    }
+  std::vector<double> diag(n), off(n, 0.0);
+  // Householder tridiagonalisation (rows n-1 .. 1)
+  for (int i = n - 1; i > 0; --i) {
+    const int l = i - 1;
+    double h = 0.0, scale = 0.0;
+    if (l > 0) {
+      for (int k = 0; k <= l; ++k) scale += std::fabs(A(i, k));
+      if (scale == 0.0) {
+        off[i] = A(i, l);
+      } else {
+        for (int k = 0; k <= l; ++k) {
+          A(i, k) /= scale;
+          h += A(i, k) * A(i, k);
+        }
+        double f = A(i, l);
+        const double g = f >= 0.0 ? -std::sqrt(h) : std::sqrt(h);
+        off[i] = scale * g;
+        h -= f * g;
+        A(i, l) = f - g;
+        f = 0.0;
+        for (int j = 0; j <= l; ++j) {
+          double gg = 0.0;
+          for (int k = 0; k <= j; ++k) gg += A(j, k) * A(i, k);
+          for (int k = j + 1; k <= l; ++k) gg += A(k, j) * A(i, k);
+          off[j] = gg / h;
+          f += off[j] * A(i, j);
+        }
+        const double hh = f / (h + h);
+        for (int j = 0; j <= l; ++j) {
+          const double fj = A(i, j);
+          const double gj = off[j] - hh * fj;
+          off[j] = gj;
+          for (int k = 0; k <= j; ++k) A(j, k) -= fj * off[k] + gj * A(i, k);
+        }
+      }
+    } else {
+      off[i] = A(i, l);
+    }
+    diag[i] = h;
+  }
+  for (int i = 0; i < n; ++i) diag[i] = A(i, i);
+  // implicit QL on the tridiagonal matrix (diag, off[1..n-1])
+  for (int i = 1; i < n; ++i) off[i - 1] = off[i];
+  off[n - 1] = 0.0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0, mm;
+    do {
+      for (mm = l; mm < n - 1; ++mm) {
+        const double dd = std::fabs(diag[mm]) + std::fabs(diag[mm + 1]);
+        if (std::fabs(off[mm]) <= 1e-15 * dd) break;
+      }
+      if (mm != l) {
+        if (++iter > 60) break;
+        double g = (diag[l + 1] - diag[l]) / (2.0 * off[l]);
+        double r = std::hypot(g, 1.0);
+        g = diag[mm] - diag[l] + off[l] / (g + (g >= 0.0 ? std::fabs(r) : -std::fabs(r)));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = mm - 1; i >= l; --i) {
+          double f = s * off[i];
+          const double b = c * off[i];
+          r = std::hypot(f, g);
+          off[i + 1] = r;
+          if (r == 0.0) {
+            diag[i + 1] -= p;
+            off[mm] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = diag[i + 1] - p;
+          r = (diag[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          diag[i + 1] = g + p;
+          g = c * r - b;
+        }
+        if (r == 0.0 && i >= l) continue;
+        diag[l] -= p;
+        off[l] = g;
+        off[mm] = 0.0;
+      }
+    } while (mm != l);
+  }
+  std::sort(diag.begin(), diag.end());
+  std::vector<double> ev(d);
+  for (int k = 0; k < d; ++k) ev[k] = diag[2 * k];
+  return ev;
+}
+
 }  // namespace yh

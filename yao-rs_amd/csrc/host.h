@@ -31,6 +31,11 @@ int channel_num_qubits(const yh_op& op);
 // is_diagonal_matrix (density_matrix.rs:125-129).
 bool is_diagonal_matrix(const Mat& m, int d);
 
+// Eigenvalues (ascending) of a Hermitian d x d matrix, as hermitian_eigenvalues
+// (density_matrix.rs:227-242) computes them with faer; host-side, for reduced
+// density matrices.
+std::vector<double> hermitian_eigenvalues(const Mat& m, int d);
+
 // One Pauli-product term, OperatorString::new applied (operator.rs:27-33).
 struct Term {
   cd coeff;

@@ -132,6 +132,12 @@ struct DmArgs {
 int launch_dm_block(double2* rho, const DmArgs& a, int K, const double2* mats, hipStream_t st);
 int launch_dm_trace(const double2* rho, uint64_t dim, double* partial, double* out, hipStream_t st);
 int launch_dm_purity(const double2* rho, uint64_t dim, double* partial, double* out, hipStream_t st);
+// partial trace: kept and traced bit positions, ascending
+struct PtrArgs {
+  int32_t nkept, ntr;
+  int32_t kept[32], tr[32];
+};
+int launch_partial_tr(const double2* rho, uint64_t dim, const PtrArgs& a, double2* out, hipStream_t st);
 int launch_dm_outer(const double2* psi, double2* rho, uint64_t dim, hipStream_t st);
 
 }  // namespace yh

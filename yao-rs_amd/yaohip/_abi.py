@@ -152,6 +152,8 @@ SIGNATURES = [
     ("yh_dm_apply_fused", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
     ("yh_dm_trace", c_int, [c_void_p, P(c_double)]),
     ("yh_dm_purity", c_int, [c_void_p, P(c_double)]),
+    ("yh_dm_partial_tr", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_void_p)]),
+    ("yh_dm_entropy", c_int, [c_void_p, P(c_double)]),
     ("yh_dm_expect", c_int, [c_void_p, c_size_t, P(c_double), P(ctypes.c_int32),
                              P(ctypes.c_int32), P(ctypes.c_int32), P(c_double)]),
     ("yh_dm_probs", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_double)]),

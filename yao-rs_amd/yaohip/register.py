@@ -252,6 +252,20 @@ class DensityMatrix:
         A.check(A.lib().yh_dm_purity(self._h, ctypes.byref(v)))
         return v.value
 
+    def partial_tr(self, traced_locs):
+        """partial_tr (density_matrix.rs:80-111); traced locs are bit positions
+        (LSB-first), as in the reference."""
+        t = np.array(list(traced_locs) or [0], dtype=np.int32)
+        h = ctypes.c_void_p()
+        A.check(A.lib().yh_dm_partial_tr(self._h, A.i32ptr(t), len(traced_locs), ctypes.byref(h)))
+        return DensityMatrix(h)
+
+    def von_neumann_entropy(self):
+        """von_neumann_entropy (density_matrix.rs:113-122)."""
+        v = ctypes.c_double()
+        A.check(A.lib().yh_dm_entropy(self._h, ctypes.byref(v)))
+        return v.value
+
 
 def expect_dm(dm, op: OperatorPolynomial):
     """expect_dm (expect.rs:38-42); sites are LSB-first bit positions."""
