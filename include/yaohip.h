@@ -45,7 +45,9 @@ enum yh_status {
   YH_ERR_UNSUPPORTED = 11,         /* valid in the reference, not on this backend */
   YH_ERR_DTYPE = 12,               /* operation not defined for the handle dtype  */
   YH_ERR_RESET_VALUE = 13,         /* measure.rs:256-261 reset value too large    */
-  YH_ERR_NO_DEVICE = 14
+  YH_ERR_NO_DEVICE = 14,
+  YH_ERR_IO = 15,                  /* state file could not be opened, read or written */
+  YH_ERR_FORMAT = 16               /* state file header rejected (state_io.rs:63-88) */
 };
 
 enum yh_dtype { YH_C128 = 0, YH_C64 = 1 };
@@ -122,6 +124,17 @@ int yh_reg_normalize(yh_reg *reg);                                /* normalize :
 int yh_reg_fidelity(const yh_reg *a, const yh_reg *b, double *out); /* fidelity :115-124 */
 /* Device address of the amplitudes (interleaved re,im; double or float by dtype). */
 int yh_reg_device_ptr(const yh_reg *reg, void **ptr);
+
+/* ---- state files: yao-state-v1 (yao-cli/src/state_io.rs) ---- */
+/* write_state (state_io.rs:18-47): header line + little-endian (re, im) f64 pairs,
+ * streamed from HBM in chunks (complex64 registers are widened). */
+int yh_state_write(const yh_reg *reg, const char *path);
+/* read_state_from_reader (state_io.rs:55-102) into a new register of `dtype`;
+ * header checks and messages as the reference (YH_ERR_FORMAT), file errors
+ * YH_ERR_IO. */
+int yh_state_read(const char *path, int dtype, yh_reg **out);
+/* Host-only: parse and check a state file's header; *num_qubits on success. */
+int yh_state_read_header(const char *path, int *num_qubits);
 
 /* ---- circuit application (src/apply.rs) ---- */
 /* Validates the element list as Circuit::new does (circuit.rs:150-272) for a

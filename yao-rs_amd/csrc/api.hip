@@ -349,23 +349,6 @@ static int lower_terms(int nbits, const std::vector<Term>& terms, bool lsb_sites
 using namespace yh;
 
 // ====================================== C ABI ======================================
-struct yh_reg {
-  int nbits = 0;
-  int dtype = YH_C128;
-  DevBuf state;
-  Workspace w;
-  size_t amp_bytes() const { return dtype == YH_C128 ? 16 : 8; }
-  size_t bytes() const { return (size_t(1) << nbits) * amp_bytes(); }
-};
-
-struct yh_dm {
-  int nbits = 0;
-  DevBuf rho;
-  Workspace w;
-  uint64_t dim() const { return uint64_t(1) << nbits; }
-  size_t bytes() const { return size_t(dim()) * dim() * 16; }
-};
-
 #define YH_GUARD(cond, st, msg) \
   do {                          \
     if (!(cond)) return fail(st, msg); \

@@ -77,3 +77,21 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]);
 
 }  // namespace yh
+
+// Register and density-matrix handles (opaque in the C ABI).
+struct yh_reg {
+  int nbits = 0;
+  int dtype = YH_C128;
+  yh::DevBuf state;
+  yh::Workspace w;
+  size_t amp_bytes() const { return dtype == YH_C128 ? 16 : 8; }
+  size_t bytes() const { return (size_t(1) << nbits) * amp_bytes(); }
+};
+
+struct yh_dm {
+  int nbits = 0;
+  yh::DevBuf rho;
+  yh::Workspace w;
+  uint64_t dim() const { return uint64_t(1) << nbits; }
+  size_t bytes() const { return size_t(dim()) * dim() * 16; }
+};

@@ -8,7 +8,8 @@ from .circuit import (Annotation, Circuit, CircuitError, Gate, NoiseChannel, Op,
                       OperatorString, PositionedChannel, PositionedGate, channel, control, label, lower,
                       put, tfim_hamiltonian)
 from .register import (ArrayReg, DensityMatrix, PostProcess, apply, apply_inplace, channel_kraus,
-                       expect_arrayreg, expect_dm, expect_grad, gate_matrix, measure_with_postprocess, probs, sample)
+                       expect_arrayreg, expect_dm, expect_grad, gate_matrix, measure_with_postprocess, probs,
+                       read_state, read_state_header, sample, write_state)
 
 YhError = _abi.YhError
 
@@ -16,5 +17,6 @@ __all__ = [
     "Annotation", "ArrayReg", "Circuit", "CircuitError", "DensityMatrix", "Gate", "NoiseChannel", "Op",
     "OperatorPolynomial", "OperatorString", "PositionedChannel", "PositionedGate", "PostProcess", "YhError",
     "apply", "apply_inplace", "channel", "channel_kraus", "control", "expect_arrayreg", "expect_dm", "expect_grad",
-    "gate_matrix", "label", "lower", "measure_with_postprocess", "probs", "put", "sample", "tfim_hamiltonian",
+    "gate_matrix", "label", "lower", "measure_with_postprocess", "probs", "put", "read_state", "read_state_header",
+    "sample", "tfim_hamiltonian", "write_state",
 ]

@@ -17,7 +17,7 @@ STATUS_NAMES = {
     0: "OK", 1: "INVALID_ARG", 2: "CONTROL_CONFIG_LEN", 3: "LOC_OUT_OF_RANGE",
     4: "OVERLAPPING_LOCS", 5: "MATRIX_SIZE", 6: "NBITS_MISMATCH", 7: "MEASURE_LOC",
     8: "CHANNEL_PARAM", 9: "HIP", 10: "OUT_OF_MEMORY", 11: "UNSUPPORTED", 12: "DTYPE",
-    13: "RESET_VALUE", 14: "NO_DEVICE",
+    13: "RESET_VALUE", 14: "NO_DEVICE", 15: "IO", 16: "FORMAT",
 }
 C128, C64 = 0, 1
 ELEM_GATE, ELEM_CHANNEL, ELEM_ANNOTATION = 0, 1, 2
@@ -125,6 +125,9 @@ SIGNATURES = [
     ("yh_reg_normalize", c_int, [c_void_p]),
     ("yh_reg_fidelity", c_int, [c_void_p, c_void_p, P(c_double)]),
     ("yh_reg_device_ptr", c_int, [c_void_p, P(c_void_p)]),
+    ("yh_state_write", c_int, [c_void_p, ctypes.c_char_p]),
+    ("yh_state_read", c_int, [ctypes.c_char_p, c_int, P(c_void_p)]),
+    ("yh_state_read_header", c_int, [ctypes.c_char_p, P(c_int)]),
     ("yh_validate", c_int, [c_int, P(YhOp), c_size_t]),
     ("yh_apply", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
     ("yh_apply_fused", c_int, [c_void_p, c_int, P(YhOp), c_size_t, c_int]),

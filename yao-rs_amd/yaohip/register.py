@@ -7,6 +7,7 @@ Mirrors yao-rs's public functions over the C ABI: `ArrayReg` (src/register.rs),
 GPU through libyaohip.so; there is no host fallback.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -101,6 +102,26 @@ class ArrayReg:
     def apply(self, circuit):
         """Register::apply (register.rs:127-139)."""
         apply_inplace(circuit, self)
+
+
+def write_state(reg, path):
+    """write_state (yao-cli/src/state_io.rs:18-47): yao-state-v1 file."""
+    A.check(A.lib().yh_state_write(reg._h, os.fsencode(path)))
+
+
+def read_state(path, dtype="c128"):
+    """read_state_from_file (yao-cli/src/state_io.rs:49-102) into a device register."""
+    A.init()
+    h = ctypes.c_void_p()
+    A.check(A.lib().yh_state_read(os.fsencode(path), A.C128 if dtype == "c128" else A.C64, ctypes.byref(h)))
+    return ArrayReg(h, dtype)
+
+
+def read_state_header(path):
+    """Host-only header check of a yao-state-v1 file; returns num_qubits."""
+    n = ctypes.c_int()
+    A.check(A.lib().yh_state_read_header(os.fsencode(path), ctypes.byref(n)))
+    return n.value
 
 
 def apply_inplace(circuit, reg, fused=0):
