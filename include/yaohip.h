@@ -146,11 +146,14 @@ int yh_validate(int nbits, const yh_op *ops, size_t nops);
 int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
 /* Same semantics as yh_apply (apply_inplace, apply.rs:123-147), with gates packed
  * into passes that each stream the state through HBM once.
- * max_fuse_qubits = 5: gates fused into 5-qubit blocks (32 x 32 complex matrices,
- *   FP64 on the host), each applied as a real-split MFMA contraction (FP64 MFMA
- *   for complex128, FP32 MFMA for complex64).
- * max_fuse_qubits in 0..4: multi-gate LDS tile passes (12 qubits per pass, gates
- *   applied in groups of up to 4 qubits; 0 = this default). */
+ * max_fuse_qubits = 5 or 0 (default): gates fused into 5-qubit blocks (32 x 32
+ *   complex matrices, FP64 on the host), each applied as a real-split MFMA
+ *   contraction (FP64 MFMA for complex128, FP32 MFMA for complex64). The planner
+ *   keeps the fewest-blocks plan of several strategies and caches it by circuit
+ *   structure (gate kinds and positions), so repeated calls only rebuild the
+ *   block matrices.
+ * max_fuse_qubits in 1..4: multi-gate LDS tile passes (12 qubits per pass, gates
+ *   applied in groups of up to 4 qubits). */
 int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
                    int max_fuse_qubits);
 /* Host-only plan of the 5-qubit block path: out[0] blocks, out[1] gates in blocks

@@ -155,7 +155,7 @@ def test_block_plan_covers_every_gate():
     A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
     blocks, inblk, single = [int(x) for x in out]
     assert inblk == len(els) and single == 0
-    assert blocks <= 80
+    assert blocks <= 50
     wide = [PositionedGate(Gate.Custom(np.eye(8), False), [0, 1, 2], [3, 4, 5], [True, True, False])]
     ol = lower(els[:10] + wide)
     A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))

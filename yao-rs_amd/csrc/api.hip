@@ -558,7 +558,7 @@ int yh_apply_fused(yh_reg* reg, int circuit_nbits, const yh_op* ops, size_t nops
   YH_GUARD(max_fuse_qubits >= 0 && max_fuse_qubits <= 5, YH_ERR_INVALID_ARG, "max_fuse_qubits must be in [0, 5]");
   YH_GUARD(circuit_nbits == reg->nbits, YH_ERR_NBITS_MISMATCH, "Register and circuit qubit count mismatch");
   YH_TRY(yh_validate(circuit_nbits, ops, nops));
-  if (max_fuse_qubits == 5) {  // 5-qubit blocks, one MFMA pass each
+  if (max_fuse_qubits == 5 || max_fuse_qubits == 0) {  // 5-qubit blocks, one MFMA pass each (default)
     if (reg->dtype == YH_C128) return apply_blocks<double>(reg->w, reg->state.p, reg->nbits, ops, nops);
     return apply_blocks<float>(reg->w, reg->state.p, reg->nbits, ops, nops);
   }

@@ -20,6 +20,7 @@
 // barrier in the loop.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "host.h"
@@ -229,9 +230,9 @@ int to_bgates(int nbits, const yh_op* ops, size_t nops, std::vector<BGate>& out)
 
 struct BStep {
   bool block = false;
-  size_t gate = 0;            // single launch: index into the gate list
+  size_t gate = 0;               // single launch: index into the gate list
   int p[kBQ] = {0, 0, 0, 0, 0};  // block: row bit j <-> memory bit p[j]
-  size_t mat = 0;             // block: index of its matrix
+  std::vector<uint32_t> members; // block: gate indices, circuit order
 };
 
 // U <- G U for a gate on local rows (bit k of the gate index <- local bit lt[k]).
@@ -264,68 +265,240 @@ void left_multiply(std::vector<cd>& U, const BGate& g, const int* lt, const std:
   }
 }
 
-int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& steps,
-                std::vector<std::vector<cd>>& mats, std::vector<BGate>& bg, size_t* gates_in_blocks) {
-  YH_TRY(to_bgates(nbits, ops, nops, bg));
-  const size_t G = bg.size();
-  const uint64_t all = nbits >= 64 ? ~uint64_t(0) : (uint64_t(1) << nbits) - 1;
-  std::vector<bool> done(G, false);
-  size_t head = 0, inblk = 0;
-  while (head < G) {
-    if (done[head]) {
-      ++head;
-      continue;
-    }
-    if (!bg[head].blockable) {
-      BStep s;
-      s.gate = head;
-      steps.push_back(s);
-      done[head] = true;
-      continue;
-    }
-    uint64_t Q = 0, B = 0;
-    std::vector<size_t> members;
-    for (size_t i = head; i < G && B != all; ++i) {
-      if (done[i]) continue;
+// ---- planner ----
+// A block is a set Q of at most 5 memory bits; it absorbs, in circuit order, every
+// remaining gate inside Q that no skipped earlier gate shares a bit with (a
+// skipped gate blocks its bits). Strategies for choosing Q at each step:
+//   0 first-come: Q grows with the qubits of the gates met in circuit order;
+//   1 windows: the best of the 5-bit windows [b, b+5), plus strategy 0's set;
+//   2 local search: from each of the first ready gates, Q grows one bit at a
+//     time by the bit that lets the most gates in, plus strategy 0's set;
+//   3 both 1 and 2.
+// "Best" is the Q that absorbs the most gates. The plan with the fewest blocks
+// over the four strategies is used.
+struct Planner {
+  const std::vector<BGate>& bg;
+  int nbits;
+  uint64_t all;
+  std::vector<uint32_t> rem;  // remaining gate indices, circuit order
+  Planner(const std::vector<BGate>& g, int n) : bg(g), nbits(n), all(n >= 64 ? ~uint64_t(0) : (uint64_t(1) << n) - 1) {}
+
+  void absorb(uint64_t Q, std::vector<uint32_t>& taken) const {
+    taken.clear();
+    uint64_t B = 0;
+    for (uint32_t i : rem) {
       const BGate& g = bg[i];
-      if (!g.blockable || (g.mask & B) || popc(Q | g.mask) > kBQ) {
+      if ((g.mask & B) || !g.blockable || (g.mask & ~Q)) {
         B |= g.mask;
+        if (B == all) break;
         continue;
       }
-      Q |= g.mask;
-      members.push_back(i);
-      done[i] = true;
+      taken.push_back(i);
     }
-    // row bits: the block's qubits ascending, padded with the lowest other bits
-    BStep s;
-    s.block = true;
-    int np = 0;
-    for (int b = 0; b < nbits && np < kBQ; ++b)
-      if ((Q >> b) & 1) s.p[np++] = b;
-    for (int b = 0; b < nbits && np < kBQ; ++b)
-      if (!((Q >> b) & 1)) s.p[np++] = b;
-    auto local = [&](int bit) {
-      for (int j = 0; j < kBQ; ++j)
-        if (s.p[j] == bit) return j;
-      return -1;
-    };
-    std::vector<cd> U(size_t(kBD) * kBD, cd{0, 0});
-    for (int r = 0; r < kBD; ++r) U[size_t(r) * kBD + r] = cd{1, 0};
-    for (size_t gi : members) {
-      const BGate& g = bg[gi];
-      int lt[kBQ];
-      for (int k = 0; k < g.nt; ++k) lt[k] = local(g.tb[k]);
-      std::vector<std::pair<int, int>> lc;
-      for (auto& c : g.ctl) lc.push_back({local(c.first), c.second});
-      left_multiply(U, g, lt, lc);
-    }
-    inblk += members.size();
-    s.mat = mats.size();
-    mats.push_back(std::move(U));
-    steps.push_back(s);
   }
-  if (gates_in_blocks) *gates_in_blocks = inblk;
+  uint64_t first_come() const {
+    uint64_t Q = 0, B = 0;
+    for (uint32_t i : rem) {
+      const BGate& g = bg[i];
+      if (!g.blockable || (g.mask & B) || popc(Q | g.mask) > kBQ)
+        B |= g.mask;
+      else
+        Q |= g.mask;
+      if (B == all) break;
+    }
+    return Q;
+  }
+  void ready(std::vector<uint32_t>& out, size_t cap) const {
+    out.clear();
+    uint64_t B = 0;
+    for (uint32_t i : rem) {
+      if (!(bg[i].mask & B) && bg[i].blockable) out.push_back(i);
+      B |= bg[i].mask;
+      if (out.size() >= cap || B == all) break;
+    }
+  }
+  uint64_t grow(uint64_t Q, std::vector<uint32_t>& tmp) const {
+    while (popc(Q) < kBQ) {
+      absorb(Q, tmp);
+      std::vector<char> in_blk;
+      uint64_t cand = 0;
+      int nc = 0;
+      size_t t = 0;
+      for (uint32_t i : rem) {
+        if (t < tmp.size() && tmp[t] == i) {
+          ++t;
+          continue;
+        }
+        const uint64_t m = bg[i].mask;
+        if ((m & Q) && popc(m | Q) <= kBQ && (m & ~Q & ~cand)) {
+          cand |= m & ~Q;
+          nc = popc(cand);
+          if (nc >= 8) break;
+        }
+      }
+      if (!cand) break;
+      uint64_t best_bit = 0;
+      size_t best_n = 0;
+      for (int b = 0; b < 64; ++b)
+        if ((cand >> b) & 1) {
+          absorb(Q | (uint64_t(1) << b), tmp);
+          if (!best_bit || tmp.size() > best_n) {
+            best_n = tmp.size();
+            best_bit = uint64_t(1) << b;
+          }
+        }
+      Q |= best_bit;
+    }
+    return Q;
+  }
+  void run(int strategy, std::vector<BStep>& steps) {
+    rem.resize(bg.size());
+    for (size_t i = 0; i < bg.size(); ++i) rem[i] = uint32_t(i);
+    std::vector<uint32_t> taken, best, tmp, rd;
+    std::vector<char> done(bg.size(), 0);
+    while (!rem.empty()) {
+      const uint32_t head = rem.front();
+      if (!bg[head].blockable) {
+        BStep s;
+        s.gate = head;
+        steps.push_back(s);
+        done[head] = 1;
+      } else {
+        std::vector<uint64_t> cands{first_come()};
+        if (strategy == 1 || strategy == 3)
+          for (int b = 0; b + kBQ <= nbits; ++b) cands.push_back(((uint64_t(1) << kBQ) - 1) << b);
+        if (strategy == 2 || strategy == 3) {
+          ready(rd, 16);
+          for (uint32_t r : rd) cands.push_back(grow(bg[r].mask, tmp));
+        }
+        uint64_t bestQ = 0;
+        best.clear();
+        for (uint64_t Q : cands) {
+          absorb(Q, taken);
+          if (taken.size() > best.size()) {
+            best.swap(taken);
+            bestQ = Q;
+          }
+        }
+        BStep s;
+        s.block = true;
+        int np = 0;
+        for (int b = 0; b < nbits && np < kBQ; ++b)
+          if ((bestQ >> b) & 1) s.p[np++] = b;
+        for (int b = 0; b < nbits && np < kBQ; ++b)
+          if (!((bestQ >> b) & 1)) s.p[np++] = b;
+        s.members = best;
+        for (uint32_t i : best) done[i] = 1;
+        steps.push_back(std::move(s));
+      }
+      size_t k = 0;
+      for (uint32_t i : rem)
+        if (!done[i]) rem[k++] = i;
+      rem.resize(k);
+    }
+  }
+};
+
+// structural key of a circuit: everything but gate parameters and matrix values
+uint64_t structure_key(int nbits, const yh_op* ops, size_t nops) {
+  uint64_t h = 1469598103934665603ull ^ uint64_t(nbits);
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  for (size_t e = 0; e < nops; ++e) {
+    const yh_op& o = ops[e];
+    mix(uint64_t(o.elem) << 40 | uint64_t(uint32_t(o.kind)) << 8 | uint64_t(o.nt) << 4 | uint64_t(o.nc));
+    for (int k = 0; k < o.nt && k < YH_MAX_TARGETS; ++k) mix(uint64_t(o.targets[k]));
+    for (int k = 0; k < o.nc && k < YH_MAX_CTRLS; ++k) mix(uint64_t(o.ctrls[k]) << 1 | uint64_t(o.ctrl_vals[k] & 1));
+  }
+  return h ^ nops;
+}
+
+struct PlanCache {
+  std::mutex mu;
+  struct Entry {
+    uint64_t key;
+    int nbits;
+    size_t nops;
+    std::vector<BStep> steps;
+  };
+  std::vector<Entry> entries;  // most recent last, at most 8
+};
+PlanCache& plan_cache() {
+  static PlanCache c;
+  return c;
+}
+
+int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& steps, std::vector<BGate>& bg) {
+  YH_TRY(to_bgates(nbits, ops, nops, bg));
+  const uint64_t key = structure_key(nbits, ops, nops);
+  {
+    PlanCache& c = plan_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    for (auto& e : c.entries)
+      if (e.key == key && e.nbits == nbits && e.nops == nops) {
+        steps = e.steps;
+        return YH_OK;
+      }
+  }
+  Planner pl(bg, nbits);
+  std::vector<BStep> bestp;
+  size_t best_blocks = SIZE_MAX;
+  for (int strategy = 0; strategy < 4; ++strategy) {
+    std::vector<BStep> cand;
+    pl.run(strategy, cand);
+    size_t nb = 0;
+    for (auto& s : cand) nb += s.block;
+    if (nb < best_blocks) {
+      best_blocks = nb;
+      bestp.swap(cand);
+    }
+  }
+  steps = bestp;
+  PlanCache& c = plan_cache();
+  std::lock_guard<std::mutex> lk(c.mu);
+  c.entries.push_back({key, nbits, nops, std::move(bestp)});
+  if (c.entries.size() > 8) c.entries.erase(c.entries.begin());
   return YH_OK;
+}
+
+// FP64 block matrix of one block step (product of its gates, embedded in 5 bits)
+void block_matrix(const BStep& s, const std::vector<BGate>& bg, std::vector<cd>& U) {
+  auto local = [&](int bit) {
+    for (int j = 0; j < kBQ; ++j)
+      if (s.p[j] == bit) return j;
+    return -1;
+  };
+  U.assign(size_t(kBD) * kBD, cd{0, 0});
+  for (int r = 0; r < kBD; ++r) U[size_t(r) * kBD + r] = cd{1, 0};
+  for (uint32_t gi : s.members) {
+    const BGate& g = bg[gi];
+    int lt[kBQ];
+    for (int k = 0; k < g.nt; ++k) lt[k] = local(g.tb[k]);
+    std::vector<std::pair<int, int>> lc;
+    for (auto& c : g.ctl) lc.push_back({local(c.first), c.second});
+    left_multiply(U, g, lt, lc);
+  }
+}
+
+// real-split [[Ur, -Ui], [Ui, Ur]] in MFMA A-fragment order: entry
+// (rt * 16 + ks) * 64 + lane holds real row 16 rt + lane % 16, column 4 ks + lane / 16
+template <class R>
+void to_fragments(const std::vector<cd>& U, R* o) {
+  constexpr int RD = 2 * kBD;
+  auto at = [&](int r, int c) -> R {
+    const bool ri = r >= kBD, ci = c >= kBD;
+    const cd u = U[size_t(r % kBD) * kBD + (c % kBD)];
+    if (!ri && !ci) return R(u.re);
+    if (!ri && ci) return R(-u.im);
+    if (ri && !ci) return R(u.im);
+    return R(u.re);
+  };
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ks = 0; ks < 16; ++ks)
+      for (int l = 0; l < 64; ++l) o[(rt * 16 + ks) * 64 + l] = at(16 * rt + l % 16, 4 * ks + l / 16);
+  (void)RD;
 }
 
 }  // namespace
@@ -335,32 +508,10 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
   using V = typename Cplx<R>::V;
   if (nbits < kBQ) return apply_gates<R>(w, state, nbits, ops, nops);
   std::vector<BStep> steps;
-  std::vector<std::vector<cd>> mats;
   std::vector<BGate> bg;
-  YH_TRY(plan_blocks(nbits, ops, nops, steps, mats, bg, nullptr));
-  // real-split matrices [[Ur, -Ui], [Ui, Ur]], row-major 64 x 64
-  constexpr int RD = 2 * kBD;
-  // stored in MFMA A-fragment order: entry (rt * 16 + ks) * 64 + lane holds
-  // real-split row 16 rt + lane % 16, column 4 ks + lane / 16
-  std::vector<R> host(mats.size() * kFrag);
-  std::vector<R> ur(RD * RD);
-  for (size_t bi = 0; bi < mats.size(); ++bi) {
-    const std::vector<cd>& U = mats[bi];
-    for (int r = 0; r < kBD; ++r)
-      for (int c = 0; c < kBD; ++c) {
-        const cd u = U[size_t(r) * kBD + c];
-        ur[r * RD + c] = R(u.re);
-        ur[r * RD + kBD + c] = R(-u.im);
-        ur[(kBD + r) * RD + c] = R(u.im);
-        ur[(kBD + r) * RD + kBD + c] = R(u.re);
-      }
-    R* o = host.data() + bi * kFrag;
-    for (int rt = 0; rt < 4; ++rt)
-      for (int ks = 0; ks < 16; ++ks)
-        for (int l = 0; l < 64; ++l) o[(rt * 16 + ks) * 64 + l] = ur[(16 * rt + l % 16) * RD + 4 * ks + l / 16];
-  }
-  YH_TRY(w.upload(host.data(), host.size() * sizeof(R), w.tiles));
-  const R* dmat = static_cast<const R*>(w.tiles.p);
+  YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
+  size_t nblk = 0;
+  for (auto& s : steps) nblk += s.block;
   // persistent grid: as many workgroups as the CUs hold at once
   static int resident[2] = {0, 0};
   int& res = resident[sizeof(R) == 8 ? 0 : 1];
@@ -372,9 +523,24 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
     YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block5<R>, 256, 0));
     res = prop.multiProcessorCount * std::max(1, per_cu);
   }
+  // Block matrices are built on the host while the GPU runs the previous blocks:
+  // each goes to its own slot of a pinned staging area and is copied with an
+  // async copy ordered before its launch (nothing else writes the area in this call).
+  const size_t slot = kFrag * sizeof(R);
+  YH_TRY(w.sync());
+  YH_TRY(w.bpin.ensure(std::max<size_t>(1, nblk) * slot));
+  YH_TRY(w.tiles.ensure(std::max<size_t>(1, nblk) * slot));
+  unsigned char* hst = static_cast<unsigned char*>(w.bpin.p);
+  unsigned char* dev = static_cast<unsigned char*>(w.tiles.p);
   V* s = static_cast<V*>(state);
+  std::vector<cd> U;
+  size_t bi = 0;
   for (const BStep& st : steps) {
     if (st.block) {
+      block_matrix(st, bg, U);
+      R* frag = reinterpret_cast<R*>(hst + bi * slot);
+      to_fragments<R>(U, frag);
+      YH_HIP(hipMemcpyAsync(dev + bi * slot, frag, slot, hipMemcpyHostToDevice, w.stream));
       BlockArgs a{};
       int sp[kBQ];
       std::memcpy(sp, st.p, sizeof(sp));
@@ -388,9 +554,10 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
       const uint64_t grid = std::min<uint64_t>((a.nchunk + 3) / 4, uint64_t(res));
       YH_TRY(w.time_begin(YH_K_FUSED));
       hipLaunchKernelGGL(k_block5<R>, dim3(unsigned(grid)), dim3(256), 0, w.stream, s, a,
-                         dmat + st.mat * kFrag);
+                         reinterpret_cast<const R*>(dev + bi * slot));
       YH_HIP(hipGetLastError());
       YH_TRY(w.time_end());
+      ++bi;
     } else {
       const BGate& g = bg[st.gate];
       yh_op one = ops[g.op_index];
@@ -410,13 +577,17 @@ template int apply_blocks<float>(Workspace&, void*, int, const yh_op*, size_t);
 // out[0] blocks, out[1] gates (X split per target) in blocks, out[2] single launches
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]) {
   std::vector<BStep> steps;
-  std::vector<std::vector<cd>> mats;
   std::vector<BGate> bg;
-  size_t inblk = 0;
-  YH_TRY(plan_blocks(nbits, ops, nops, steps, mats, bg, &inblk));
-  out[0] = int64_t(mats.size());
-  out[1] = int64_t(inblk);
-  out[2] = int64_t(steps.size() - mats.size());
+  YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
+  out[0] = out[1] = out[2] = 0;
+  for (auto& s : steps) {
+    if (s.block) {
+      out[0] += 1;
+      out[1] += int64_t(s.members.size());
+    } else {
+      out[2] += 1;
+    }
+  }
   return YH_OK;
 }
 

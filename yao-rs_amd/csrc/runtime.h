@@ -41,6 +41,7 @@ struct Workspace {
   DevBuf shots_out;  // sampled outcomes
   DevBuf tiles;      // tile-pass groups and micro gates
   PinBuf pin;
+  PinBuf bpin;       // block matrices staged for async copies (yh_apply_fused)
   // launch timing (measurement hooks): events around each launch while enabled
   struct TimedLaunch {
     int kclass;
