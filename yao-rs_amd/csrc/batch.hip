@@ -1,18 +1,20 @@
 // batch.hip — batched VQE energies: thousands of hardware-efficient ansatz
-// instances of uneven qubit counts in one persistent launch.
+// instances of uneven qubit counts in a few launches, replayed from a hipGraph.
 //
-// Ansatz (per layer): Ry then Rz on every qubit, then CZ(i, i+1) for i < n-1
-// (CZ = control([i], [i+1], Z), yao-rs easybuild.rs:516), from |0...0>.
-// Energy: H = -J sum_i Z_i Z_{i+1} - h sum_i X_i (open chain), FP64 reductions.
+// Ansatz (per layer): Ry then Rz (or Rz, Ry, Rz) on every qubit, then CZ(i, i+1)
+// for i < n-1 (CZ = control([i], [i+1], Z), yao-rs easybuild.rs:516), from
+// |0...0>. Energy: H = -J sum_i Z_i Z_{i+1} - h sum_i X_i (open chain), FP64
+// reductions.
 //
-// Each workgroup pulls instances from an atomic queue (largest first). The
-// per-qubit Rz*Ry product of a layer is composed in FP64 on the host, so a layer
-// is n single-qubit 2x2 updates followed by the CZ chain, which is the exact sign
-// (-1)^popcount(x & x>>1) and is applied in the same register pass as the last
-// rotation group. A state of at most 2^TB amplitudes lives in LDS for the whole
-// instance; a larger one streams through the workgroup's HBM slot in tiles of
-// 2^TB amplitudes: one pass holds bits 0..TB-1, the others bits 0..4 plus a chunk
-// of the high bits, so a layer costs 1 + ceil((n - TB) / (TB - 5)) passes.
+// The per-qubit gates of a layer are composed in FP64 on the host, so a layer is n
+// single-qubit 2x2 updates followed by the CZ chain, the exact sign
+// (-1)^popcount(x & x>>1), applied in the register pass of the last rotation group.
+// Instances with at most 2^TB amplitudes live in LDS for their whole evaluation:
+// one persistent kernel whose workgroups pull them from an atomic queue. Larger
+// instances keep their states in HBM and advance one step per launch (a step is
+// one tile pass of one layer), every (instance, 2^TB-amplitude work item) being its
+// own workgroup, so the whole GPU streams all large states at once; a layer costs
+// 1 + ceil((n - TB) / (TB - 5)) passes.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -101,20 +103,12 @@ __device__ __forceinline__ double bred(double v, double* sh) {
   return r;
 }
 
-// memory bit of tile position p for a pass whose high positions map to hb[]
-__device__ __forceinline__ uint64_t tile_addr(int p, const uint8_t* hb, int nh) {
-  uint64_t a = uint64_t(p & ((1 << kVqeLow) - 1));
-  for (int k = 0; k < nh; ++k) a |= uint64_t((p >> (kVqeLow + k)) & 1) << hb[k];
-  return a;
-}
-
-// BIG = false: instances [first, last) fit in LDS (n <= TB); BIG = true: they stream
-// through the workgroup's HBM slot. Two instantiations keep the register budgets apart.
-template <class R, int TB, bool BIG>
-__global__ __launch_bounds__(256, 2) void k_vqe(const VqeInst* __restrict__ insts, int first, int last,
+// Instances [first, last) with n <= TB: the whole state stays in LDS. Each
+// workgroup pulls instances from the queue `counter`.
+template <class R, int TB>
+__global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __restrict__ insts, int first, int last,
                                                 int* __restrict__ counter,
                                                 const typename Cplx<R>::V* __restrict__ rot,
-                                                typename Cplx<R>::V* __restrict__ scratch, uint64_t slot_amps,
                                                 double* __restrict__ energies, double J, double h) {
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
@@ -122,7 +116,6 @@ __global__ __launch_bounds__(256, 2) void k_vqe(const VqeInst* __restrict__ inst
   __shared__ double red[16];
   __shared__ int cur;
   const int tid = threadIdx.x, nt = blockDim.x;
-  V* slot = scratch + uint64_t(blockIdx.x) * slot_amps;
   for (;;) {
     __syncthreads();
     if (tid == 0) cur = first + atomicAdd(counter, 1);
@@ -133,201 +126,219 @@ __global__ __launch_bounds__(256, 2) void k_vqe(const VqeInst* __restrict__ inst
     const int n = in.n, depth = in.depth;
     const V* rt = rot + in.rot_off;
     double zz_acc = 0.0, x_acc = 0.0;
-    if constexpr (!BIG) {
-      // ---------------- whole state in LDS ----------------
-      const int N = 1 << n;
-      for (int x = tid; x < N; x += nt) lds[vpad<TB>(x)] = cmk<V>(R(x == 0 ? 1 : 0), R(0));
-      __syncthreads();
-      for (int layer = 0; layer < depth; ++layer) {
-        const V* ul = rt + int64_t(layer) * n * 4;
-        for (int g0 = 0; g0 < n; g0 += 4) {
-          const int gb = min(4, n - g0);
-          const bool last = g0 + 4 >= n;
-          const int nsub = N >> gb;
-          for (int sc = tid; sc < nsub; sc += nt) {
-            const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-            V a[16];
+    const int N = 1 << n;
+    for (int x = tid; x < N; x += nt) lds[vpad<TB>(x)] = cmk<V>(R(x == 0 ? 1 : 0), R(0));
+    lds_barrier();
+    for (int layer = 0; layer < depth; ++layer) {
+      const V* ul = rt + int64_t(layer) * n * 4;
+      for (int g0 = 0; g0 < n; g0 += 4) {
+        const int gb = min(4, n - g0);
+        const bool last = g0 + 4 >= n;
+        const int nsub = N >> gb;
+        for (int sc = tid; sc < nsub; sc += nt) {
+          const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+          V a[16];
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-              if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
-            rot_group<R>(a, gb, ul + g0 * 4);
+          for (int i = 0; i < 16; ++i)
+            if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+          rot_group<R>(a, gb, ul + g0 * 4);
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
-              if (i < (1 << gb)) {
-                const int x = base + (i << g0);
-                V v = a[i];
-                if (last && cz_sign(uint64_t(x), n) < 0) {
-                  v.x = -v.x;
-                  v.y = -v.y;
-                }
-                lds[vpad<TB>(x)] = v;
+          for (int i = 0; i < 16; ++i)
+            if (i < (1 << gb)) {
+              const int x = base + (i << g0);
+              V v = a[i];
+              if (last && cz_sign(uint64_t(x), n) < 0) {
+                v.x = -v.x;
+                v.y = -v.y;
               }
-          }
-          __syncthreads();
+              lds[vpad<TB>(x)] = v;
+            }
         }
+        lds_barrier();
       }
-      for (int x = tid; x < N; x += nt) {
-        const V p = lds[vpad<TB>(x)];
-        const double pr = double(p.x), pi = double(p.y);
-        zz_acc += (pr * pr + pi * pi) * zz_of(uint64_t(x), n);
-        for (int b = 0; b < n; ++b) {
-          const V q = lds[vpad<TB>(x ^ (1 << b))];
-          x_acc += pr * double(q.x) + pi * double(q.y);
-        }
-      }
-    } else {
-      // ---------------- state streams through the HBM slot ----------------
-      const int nchunk = (n - TB + (TB - kVqeLow) - 1) / (TB - kVqeLow);
-      const int npass = 1 + nchunk;
-      const uint64_t ntile = uint64_t(1) << (n - TB);
-      __shared__ uint8_t hb[TB];
-      __shared__ uint8_t lkpos[TB];
-      for (int layer = 0; layer <= depth; ++layer) {  // layer == depth: energy passes (read only)
-        const V* ul = rt + int64_t(layer < depth ? layer : 0) * n * 4;
-        for (int ps = 0; ps < npass; ++ps) {
-          // tile bits: pass 0 -> 0..TB-1; pass k -> 0..4 plus chunk k of [TB, n)
-          int nh, lo = 0;
-          if (tid == 0) {
-            if (ps == 0) {
-              for (int k = 0; k < TB - kVqeLow; ++k) hb[k] = uint8_t(kVqeLow + k);
-            } else {
-              lo = TB + (ps - 1) * (TB - kVqeLow);
-              for (int k = 0; k < TB - kVqeLow; ++k) hb[k] = uint8_t(min(lo + k, n - 1));
-            }
-          }
-          __syncthreads();
-          if (ps == 0) nh = TB - kVqeLow;
-          else {
-            lo = TB + (ps - 1) * (TB - kVqeLow);
-            nh = min(TB - kVqeLow, n - lo);
-          }
-          const int tbits = kVqeLow + nh;  // tile qubits in this pass
-          // sorted locked bits of the tile (for the tile base)
-          if (tid == 0) {
-            int c = 0;
-            for (int b = 0; b < kVqeLow; ++b) lkpos[c++] = uint8_t(b);
-            for (int k = 0; k < nh; ++k) lkpos[c++] = hb[k];
-            for (int a = 1; a < c; ++a)
-              for (int b2 = a; b2 > 0 && lkpos[b2 - 1] > lkpos[b2]; --b2) {
-                uint8_t t = lkpos[b2 - 1];
-                lkpos[b2 - 1] = lkpos[b2];
-                lkpos[b2] = t;
-              }
-          }
-          __syncthreads();
-          const int TNp = 1 << tbits;
-          const uint64_t ntp = uint64_t(1) << (n - tbits);
-          // rotation bits handled by this pass: pass 0 all its bits, others only the chunk
-          const int rb_first = ps == 0 ? 0 : kVqeLow;
-          const bool last_pass = ps == npass - 1;
-          // thread tid owns tile positions tid + 256 k; their offsets inside a tile are
-          // the same for every tile of the pass
-          constexpr int kPer = (1 << TB) / 256;
-          uint32_t off[kPer];
-#pragma unroll
-          for (int k = 0; k < kPer; ++k) {
-            const int p = tid + k * 256;
-            off[k] = p < TNp ? uint32_t(tile_addr(p, hb, nh)) : 0u;
-          }
-          // |0...0> is generated, not read, until the first pass has written the slot
-          const bool gen0 = depth == 0 || (layer == 0 && ps == 0);
-          auto tile_base = [&](uint64_t t) {
-            uint64_t tb = t;
-            for (int k = 0; k < tbits; ++k) {
-              const int pp = lkpos[k];
-              tb = ((tb >> pp) << (pp + 1)) | (tb & ((uint64_t(1) << pp) - 1));
-            }
-            return tb;
-          };
-          // the next tile's amplitudes are fetched into registers while this tile computes
-          V pre[kPer];
-          auto fetch = [&](uint64_t t) {
-            const uint64_t tbn = tile_base(t);
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-              const int p = tid + k * 256;
-              if (p < TNp) {
-                const uint64_t x = tbn + off[k];
-                pre[k] = gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : nt_load(&slot[x]);
-              }
-            }
-          };
-          fetch(0);
-          for (uint64_t t = 0; t < ntp; ++t) {
-            const uint64_t tb = tile_base(t);
-#pragma unroll
-            for (int k = 0; k < kPer; ++k) {
-              const int p = tid + k * 256;
-              if (p < TNp) lds[vpad<TB>(p)] = pre[k];
-            }
-            __syncthreads();
-            if (t + 1 < ntp) fetch(t + 1);
-            if (layer < depth) {
-              for (int g0 = rb_first; g0 < tbits; g0 += 4) {
-                const int gb = min(4, tbits - g0);
-                const bool lastg = g0 + 4 >= tbits;
-                const int nsub = TNp >> gb;
-                // matrices of the group's memory bits
-                // memory bit of each local qubit, made wave-uniform so the 2x2
-                // matrices come in through scalar loads
-                int mbk[4];
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                  const int pos = g0 + k;
-                  const int mb = k < gb ? (pos < kVqeLow ? pos : int(hb[pos - kVqeLow])) : 0;
-                  mbk[k] = __builtin_amdgcn_readfirstlane(mb);
-                }
-                for (int sc = tid; sc < nsub; sc += nt) {
-                  const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-                  V a[16];
-#pragma unroll
-                  for (int i = 0; i < 16; ++i)
-                    if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
-#pragma unroll
-                  for (int k = 0; k < 4; ++k)
-                    if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
-#pragma unroll
-                  for (int i = 0; i < 16; ++i)
-                    if (i < (1 << gb)) {
-                      const int p = base + (i << g0);
-                      V v = a[i];
-                      if (last_pass && lastg && cz_sign(tb + tile_addr(p, hb, nh), n) < 0) {
-                        v.x = -v.x;
-                        v.y = -v.y;
-                      }
-                      lds[vpad<TB>(p)] = v;
-                    }
-                }
-                __syncthreads();
-              }
-#pragma unroll
-              for (int k = 0; k < kPer; ++k) {
-                const int p = tid + k * 256;
-                if (p < TNp) nt_store(lds[vpad<TB>(p)], &slot[tb + off[k]]);
-              }
-            } else {
-              // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
-              for (int p = tid; p < TNp; p += nt) {
-                const V v = lds[vpad<TB>(p)];
-                const double pr = double(v.x), pi = double(v.y);
-                if (ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(tb + tile_addr(p, hb, nh), n);
-                for (int pos = rb_first; pos < tbits; ++pos) {
-                  const V q = lds[vpad<TB>(p ^ (1 << pos))];
-                  x_acc += pr * double(q.x) + pi * double(q.y);
-                }
-              }
-            }
-            __syncthreads();
-          }
-          __threadfence();
-          __syncthreads();
-        }
+    }
+    for (int x = tid; x < N; x += nt) {
+      const V p = lds[vpad<TB>(x)];
+      const double pr = double(p.x), pi = double(p.y);
+      zz_acc += (pr * pr + pi * pi) * zz_of(uint64_t(x), n);
+      for (int b = 0; b < n; ++b) {
+        const V q = lds[vpad<TB>(x ^ (1 << b))];
+        x_acc += pr * double(q.x) + pi * double(q.y);
       }
     }
     const double zz = bred(zz_acc, red);
     const double xs = bred(x_acc, red);
     if (tid == 0) energies[in.out] = -J * zz - h * xs;
   }
+}
+
+}  // namespace
+
+// Instances with n > TB: their states live in HBM (one buffer each) and every
+// instance advances one step per launch. Step k of an instance is pass
+// k % npass of layer k / npass (layer == depth: the energy passes). Pass 0 holds
+// memory bits 0..TB-1; pass c > 0 holds bits 0..4 plus the c-th of npass - 1 equal
+// chunks of bits TB..n-1, so a pass's tile has tbits <= TB bits and a work item
+// (one workgroup) takes 2^(TB - tbits) consecutive tiles: 2^TB amplitudes, and
+// 2^(n - TB) items per instance in every step.
+struct VqeStep {
+  int32_t inst;       // sorted instance index
+  int32_t step;       // step of that instance
+  int64_t item_base;  // first work item of this entry within the launch
+};
+struct VqeBig {
+  int64_t state_off;  // amplitude offset of the instance's state
+  int64_t part_off;   // offset, in (zz, x) pairs, of its energy partials
+};
+
+__host__ __device__ inline int vqe_npass(int n, int tb) {
+  return n <= tb ? 1 : 1 + (n - tb + (tb - kVqeLow) - 1) / (tb - kVqeLow);
+}
+
+namespace {
+
+template <class R, int TB>
+__global__ __launch_bounds__(256) void k_vqe_step(const VqeInst* __restrict__ insts, const VqeBig* __restrict__ big,
+                                                  const VqeStep* __restrict__ ent, int nent,
+                                                  const typename Cplx<R>::V* __restrict__ rot,
+                                                  typename Cplx<R>::V* __restrict__ states,
+                                                  double* __restrict__ partials) {
+  using V = typename Cplx<R>::V;
+  constexpr int TN = 1 << TB;
+  constexpr int kPer = TN / 256;
+  __shared__ V lds[TN + TN / 16];
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  // the entry owning this workgroup (the same for every thread)
+  const int64_t w = blockIdx.x;
+  int lo = 0, hi = nent - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ent[mid].item_base <= w) lo = mid;
+    else hi = mid - 1;
+  }
+  const VqeStep e = ent[lo];
+  const uint64_t item = uint64_t(w - e.item_base);
+  const VqeInst in = insts[e.inst];
+  const VqeBig bg = big[e.inst];
+  const int n = in.n, depth = in.depth;
+  const int npass = vqe_npass(n, TB);
+  const int layer = e.step / npass, ps = e.step % npass;
+  // pass geometry: tile positions 0..4 are memory bits 0..4, positions 5..tbits-1
+  // the contiguous memory bits lo_bit..lo_bit+nh-1
+  int nh, lo_bit;
+  if (ps == 0) {
+    nh = TB - kVqeLow;
+    lo_bit = kVqeLow;
+  } else {
+    const int cs = (n - TB + npass - 2) / (npass - 1);
+    lo_bit = TB + (ps - 1) * cs;
+    nh = min(cs, n - lo_bit);
+  }
+  const int tbits = kVqeLow + nh;
+  const int tmask = (1 << tbits) - 1;
+  const uint64_t t0 = item << (TB - tbits);
+  // memory index of LDS position p: tile t0 + (p >> tbits) fills the bits outside
+  // the tile in ascending order (zeros inserted at 0..4 and at the chunk)
+  auto mem_index = [&](int p) {
+    const uint64_t t = t0 + uint64_t(p >> tbits);
+    uint64_t y = t << kVqeLow;
+    y = ((y >> lo_bit) << (lo_bit + nh)) | (y & ((uint64_t(1) << lo_bit) - 1));
+    const uint64_t q = uint64_t(p & tmask);
+    return y | (q & ((1u << kVqeLow) - 1)) | ((q >> kVqeLow) << lo_bit);
+  };
+  V* st = states + bg.state_off;
+  const bool gen0 = depth == 0 || (layer == 0 && ps == 0);
+  uint64_t xs[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int p = tid + 256 * k;
+    xs[k] = mem_index(p);
+    lds[vpad<TB>(p)] = gen0 ? cmk<V>(R(xs[k] == 0 ? 1 : 0), R(0)) : st[xs[k]];
+  }
+  lds_barrier();
+  const int rb_first = ps == 0 ? 0 : kVqeLow;
+  if (layer < depth) {
+    const V* ul = rot + in.rot_off + int64_t(layer) * n * 4;
+    const bool last_pass = ps == npass - 1;
+    for (int g0 = rb_first; g0 < tbits; g0 += 4) {
+      const int gb = min(4, tbits - g0);
+      const bool lastg = g0 + 4 >= tbits;
+      const int nsub = TN >> gb;
+      int mbk[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int pos = g0 + k;
+        const int mb = k < gb ? (pos < kVqeLow ? pos : lo_bit + pos - kVqeLow) : 0;
+        mbk[k] = __builtin_amdgcn_readfirstlane(mb);
+      }
+      for (int sc = tid; sc < nsub; sc += 256) {
+        const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+        V a[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+          if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
+#pragma unroll
+        for (int i = 0; i < 16; ++i)
+          if (i < (1 << gb)) {
+            const int p = base + (i << g0);
+            V v = a[i];
+            if (last_pass && lastg && cz_sign(mem_index(p), n) < 0) {
+              v.x = -v.x;
+              v.y = -v.y;
+            }
+            lds[vpad<TB>(p)] = v;
+          }
+      }
+      lds_barrier();
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) st[xs[k]] = lds[vpad<TB>(tid + 256 * k)];
+  } else {
+    // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
+    double zz_acc = 0.0, x_acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + 256 * k;
+      const V v = lds[vpad<TB>(p)];
+      const double pr = double(v.x), pi = double(v.y);
+      if (ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(xs[k], n);
+      for (int pos = rb_first; pos < tbits; ++pos) {
+        const V q = lds[vpad<TB>(p ^ (1 << pos))];
+        x_acc += pr * double(q.x) + pi * double(q.y);
+      }
+    }
+    const double zz = bred(zz_acc, red);
+    const double xx = bred(x_acc, red);
+    if (tid == 0) {
+      double* o = partials + 2 * (bg.part_off + int64_t(ps) * (int64_t(1) << (n - TB)) + int64_t(item));
+      o[0] = zz;
+      o[1] = xx;
+    }
+  }
+}
+
+// energies of the HBM-resident instances: one workgroup per instance folds its
+// npass * 2^(n - TB) partial pairs in a fixed order
+template <int TB>
+__global__ __launch_bounds__(256) void k_vqe_energy(const VqeInst* __restrict__ insts, const VqeBig* __restrict__ big,
+                                                    const double* __restrict__ partials, double* __restrict__ energies,
+                                                    double J, double h) {
+  __shared__ double red[16];
+  const VqeInst in = insts[blockIdx.x];
+  const VqeBig bg = big[blockIdx.x];
+  const int64_t np = int64_t(vqe_npass(in.n, TB)) << (in.n - TB);
+  double zz = 0.0, xx = 0.0;
+  for (int64_t k = threadIdx.x; k < np; k += 256) {
+    zz += partials[2 * (bg.part_off + k)];
+    xx += partials[2 * (bg.part_off + k) + 1];
+  }
+  const double z = bred(zz, red);
+  const double x = bred(xx, red);
+  if (threadIdx.x == 0) energies[in.out] = -J * z - h * x;
 }
 
 __global__ void k_reset_counter(int* c) {
@@ -365,7 +376,6 @@ static void compose_block(const double* a, int apq, cd u[4]) {
 struct VqeConfig {
   int tile_bits = 12;
   int blocks_per_cu = 2;
-  int threads = 256;
 };
 
 }  // namespace yh
@@ -376,14 +386,21 @@ struct yh_vqe_batch {
   int max_instances = 0, max_qubits = 0, dtype = YH_C64;
   int cus = 256;
   Workspace w;
-  DevBuf insts, rot, scratch, energies, counter;
+  DevBuf insts, rot, energies, counter, big, steps, states, partials;
   VqeConfig cfg;
   bool tuned = false;
-  std::vector<int> sorted_n;  // qubit counts in upload order (descending)
+  std::vector<int> sorted_n, sorted_d;  // instance shapes in upload order (n descending)
+  // step plan of the HBM-resident instances for one tile size
+  struct Launch {
+    int64_t entry_off, nentries, items;
+  };
+  int plan_tb = -1;
+  uint64_t plan_key = 0;
+  int nbig = 0;
+  std::vector<Launch> launches;
   hipGraphExec_t graph = nullptr;
-  int graph_ninst = -1, graph_nbig = -1;
+  uint64_t graph_key = 0;
   double graph_J = 0, graph_h = 0;
-  VqeConfig graph_cfg;
   ~yh_vqe_batch() {
     if (graph) (void)hipGraphExecDestroy(graph);
   }
@@ -391,57 +408,106 @@ struct yh_vqe_batch {
 
 namespace yh {
 
-template <class R>
-static int vqe_launch(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
-  using V = typename Cplx<R>::V;
-  const int grid = b->cus * cfg.blocks_per_cu;
-  const uint64_t slot = uint64_t(1) << b->max_qubits;
-  // instances are sorted by qubit count, largest first: [0, nbig) stream through HBM
+static uint64_t vqe_shape_key(const yh_vqe_batch* b, int ninst, int tb) {
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ull;
+  };
+  mix(uint64_t(ninst));
+  mix(uint64_t(tb));
+  for (int i = 0; i < ninst; ++i) mix(uint64_t(b->sorted_n[i]) << 32 | uint64_t(uint32_t(b->sorted_d[i])));
+  return h;
+}
+
+// Step plan of the instances with n > tb (a prefix of the sorted table): state and
+// partial offsets, and for every step the instances still running and their first
+// work item. Uploads happen here, never inside a graph capture.
+static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
+  const uint64_t key = vqe_shape_key(b, ninst, tb);
+  if (b->plan_tb == tb && b->plan_key == key) return YH_OK;
   int nbig = 0;
-  while (nbig < ninst && b->sorted_n[nbig] > cfg.tile_bits) ++nbig;
+  while (nbig < ninst && b->sorted_n[nbig] > tb) ++nbig;
+  std::vector<VqeBig> big(std::max(1, nbig));
+  int64_t soff = 0, poff = 0;
+  int smax = 0;
+  for (int i = 0; i < nbig; ++i) {
+    const int n = b->sorted_n[i];
+    big[i].state_off = soff;
+    big[i].part_off = poff;
+    soff += int64_t(1) << n;
+    poff += int64_t(vqe_npass(n, tb)) << (n - tb);
+    smax = std::max(smax, (b->sorted_d[i] + 1) * vqe_npass(n, tb));
+  }
+  std::vector<VqeStep> ent;
+  std::vector<yh_vqe_batch::Launch> L;
+  for (int k = 0; k < smax; ++k) {
+    yh_vqe_batch::Launch l{int64_t(ent.size()), 0, 0};
+    for (int i = 0; i < nbig; ++i) {
+      const int n = b->sorted_n[i];
+      if (k < (b->sorted_d[i] + 1) * vqe_npass(n, tb)) {
+        ent.push_back(VqeStep{i, k, l.items});
+        l.items += int64_t(1) << (n - tb);
+        ++l.nentries;
+      }
+    }
+    L.push_back(l);
+  }
+  const size_t amp = b->dtype == YH_C128 ? 16 : 8;
+  YH_TRY(b->states.ensure(std::max<int64_t>(soff, 1) * amp));
+  YH_TRY(b->partials.ensure(std::max<int64_t>(poff, 1) * 2 * sizeof(double)));
+  YH_TRY(b->w.upload(big.data(), big.size() * sizeof(VqeBig), b->big));
+  if (!ent.empty()) YH_TRY(b->w.upload(ent.data(), ent.size() * sizeof(VqeStep), b->steps));
+  YH_TRY(b->w.sync());
+  b->nbig = nbig;
+  b->launches = L;
+  b->plan_tb = tb;
+  b->plan_key = key;
+  return YH_OK;
+}
+
+template <class R, int TB>
+static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
+  using V = typename Cplx<R>::V;
   int* ctr = static_cast<int*>(b->counter.p);
   const VqeInst* ins = static_cast<const VqeInst*>(b->insts.p);
   const V* rot = static_cast<const V*>(b->rot.p);
-  V* scr = static_cast<V*>(b->scratch.p);
   double* en = static_cast<double*>(b->energies.p);
+  const VqeBig* big = static_cast<const VqeBig*>(b->big.p);
+  const VqeStep* ent = static_cast<const VqeStep*>(b->steps.p);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, ctr);
-  if (cfg.tile_bits == 12) {
-    if (nbig > 0)
-      hipLaunchKernelGGL((k_vqe<R, 12, true>), dim3(grid), dim3(256), 0, b->w.stream, ins, 0, nbig, ctr, rot, scr,
-                         slot, en, J, h);
-    if (nbig < ninst)
-      hipLaunchKernelGGL((k_vqe<R, 12, false>), dim3(grid), dim3(256), 0, b->w.stream, ins, nbig, ninst, ctr + 1,
-                         rot, scr, slot, en, J, h);
-  } else {
-    if (nbig > 0)
-      hipLaunchKernelGGL((k_vqe<R, 13, true>), dim3(grid), dim3(256), 0, b->w.stream, ins, 0, nbig, ctr, rot, scr,
-                         slot, en, J, h);
-    if (nbig < ninst)
-      hipLaunchKernelGGL((k_vqe<R, 13, false>), dim3(grid), dim3(256), 0, b->w.stream, ins, nbig, ninst, ctr + 1,
-                         rot, scr, slot, en, J, h);
-  }
+  if (b->nbig < ninst)
+    hipLaunchKernelGGL((k_vqe<R, TB>), dim3(b->cus * cfg.blocks_per_cu), dim3(256), 0, b->w.stream, ins, b->nbig,
+                       ninst, ctr + 1, rot, en, J, h);
+  for (const auto& l : b->launches)
+    hipLaunchKernelGGL((k_vqe_step<R, TB>), dim3(unsigned(l.items)), dim3(256), 0, b->w.stream, ins, big,
+                       ent + l.entry_off, int(l.nentries), rot, static_cast<V*>(b->states.p),
+                       static_cast<double*>(b->partials.p));
+  if (b->nbig > 0)
+    hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
+                       static_cast<const double*>(b->partials.p), en, J, h);
+  YH_HIP(hipGetLastError());
   return YH_OK;
 }
 
 static int vqe_run(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
-  if (b->dtype == YH_C128) return vqe_launch<double>(b, ninst, J, h, cfg);
-  return vqe_launch<float>(b, ninst, J, h, cfg);
+  const bool c128 = b->dtype == YH_C128;
+  if (cfg.tile_bits == 11) return c128 ? vqe_launch_tb<double, 11>(b, ninst, J, h, cfg)
+                                       : vqe_launch_tb<float, 11>(b, ninst, J, h, cfg);
+  if (cfg.tile_bits == 12) return c128 ? vqe_launch_tb<double, 12>(b, ninst, J, h, cfg)
+                                       : vqe_launch_tb<float, 12>(b, ninst, J, h, cfg);
+  return c128 ? vqe_launch_tb<double, 13>(b, ninst, J, h, cfg) : vqe_launch_tb<float, 13>(b, ninst, J, h, cfg);
 }
 
-static std::vector<VqeConfig> vqe_candidates(int dtype) {
+static std::vector<VqeConfig> vqe_candidates() {
   std::vector<VqeConfig> c;
-  for (int tb : {12, 13}) {
+  for (int tb : {11, 12, 13})
     for (int bpc : {1, 2, 4}) {
-      if (dtype == YH_C128 && tb == 13) continue;  // 139 KB of LDS and a 128-VGPR prefetch buffer
-      for (int th : {256}) {
-        VqeConfig v;
-        v.tile_bits = tb;
-        v.blocks_per_cu = bpc;
-        v.threads = th;
-        c.push_back(v);
-      }
+      VqeConfig v;
+      v.tile_bits = tb;
+      v.blocks_per_cu = bpc;
+      c.push_back(v);
     }
-  }
   return c;
 }
 
@@ -462,13 +528,10 @@ int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_bat
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, 0) == hipSuccess) b->cus = prop.multiProcessorCount;
   b->cfg.tile_bits = dtype == YH_C64 ? 13 : 12;
-  const size_t amp = dtype == YH_C128 ? 16 : 8;
-  const int max_grid = b->cus * 4;
   int st = b->w.init();
   if (st == YH_OK) st = b->insts.ensure(sizeof(VqeInst) * max_instances);
   if (st == YH_OK) st = b->energies.ensure(sizeof(double) * max_instances);
   if (st == YH_OK) st = b->counter.ensure(256);
-  if (st == YH_OK) st = b->scratch.ensure(amp * (size_t(1) << max_qubits) * max_grid);
   if (st != YH_OK) {
     delete b;
     return st;
@@ -504,8 +567,8 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
       }
     if (d == 0) rot.insert(rot.end(), 4, cd{0, 0});  // keep rot_off valid
   }
-  // qubit count descending (so any tile size splits the table into a streamed prefix
-  // and an LDS-resident suffix), then cost descending (longest-processing-time order)
+  // qubit count descending (so any tile size splits the table into an HBM-resident
+  // prefix and an LDS-resident suffix), then cost descending (longest-processing-time order)
   std::vector<int> order(ninst);
   std::iota(order.begin(), order.end(), 0);
   auto cost = [&](int i) { return double(int64_t(1) << ins[i].n) * double(ins[i].depth + 1) * ins[i].n; };
@@ -515,9 +578,11 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
   });
   std::vector<VqeInst> sorted(ninst);
   b->sorted_n.resize(ninst);
+  b->sorted_d.resize(ninst);
   for (int k = 0; k < ninst; ++k) {
     sorted[k] = ins[order[k]];
     b->sorted_n[k] = sorted[k].n;
+    b->sorted_d[k] = sorted[k].depth;
   }
   YH_TRY(b->w.upload(sorted.data(), sorted.size() * sizeof(VqeInst), b->insts));
   if (b->dtype == YH_C128) {
@@ -540,9 +605,11 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   if (ninst < 1 || ninst > b->max_instances) return fail(YH_ERR_INVALID_ARG, "instance count out of range");
   YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles, apq));
   if (!b->tuned) {
-    // autotune: time every candidate launch shape on this batch, keep the fastest
+    // autotune: time every candidate (tile bits, LDS-kernel workgroups per CU) on
+    // this batch, keep the fastest
     double best = 1e30;
-    for (const VqeConfig& c : vqe_candidates(b->dtype)) {
+    for (const VqeConfig& c : vqe_candidates()) {
+      YH_TRY(vqe_prepare(b, ninst, c.tile_bits));
       YH_TRY(vqe_run(b, ninst, J, h, c));  // warm
       YH_TRY(b->w.sync());
       auto t0 = std::chrono::steady_clock::now();
@@ -556,12 +623,14 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
     }
     b->tuned = true;
   }
-  // replay a captured graph of (counter reset, persistent kernels) for this shape; the
-  // split between streamed and LDS-resident instances is part of the shape
-  int nbig = 0;
-  while (nbig < ninst && b->sorted_n[nbig] > b->cfg.tile_bits) ++nbig;
-  if (!b->graph || b->graph_ninst != ninst || b->graph_nbig != nbig || b->graph_cfg.tile_bits != b->cfg.tile_bits ||
-      b->graph_cfg.blocks_per_cu != b->cfg.blocks_per_cu) {
+  YH_TRY(vqe_prepare(b, ninst, b->cfg.tile_bits));
+  // replay a captured graph of the launch sequence; the key covers the instance
+  // shapes, the configuration and every device buffer the launches read or write
+  uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56);
+  for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->states.p,
+                        b->partials.p})
+    key = (key ^ uint64_t(reinterpret_cast<uintptr_t>(p))) * 1099511628211ull;
+  if (!b->graph || b->graph_key != key) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     YH_HIP(hipStreamSynchronize(b->w.stream));
@@ -574,9 +643,7 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
     e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(YH_ERR_HIP, std::string("graph instantiate failed: ") + hipGetErrorString(e));
-    b->graph_ninst = ninst;
-    b->graph_nbig = nbig;
-    b->graph_cfg = b->cfg;
+    b->graph_key = key;
     b->graph_J = J;
     b->graph_h = h;
   }

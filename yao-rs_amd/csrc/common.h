@@ -74,6 +74,16 @@ __device__ __forceinline__ uint64_t insert_zeros(uint64_t k, const uint8_t* pos,
   return k;
 }
 
+// Workgroup barrier that orders LDS accesses only. __syncthreads() also waits for
+// every outstanding global load and store of the calling wave, which would make a
+// register prefetch issued before the barrier complete at the barrier; here global
+// accesses are waited for only where their registers are used.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Device limits used by the launchers.
 constexpr int kMaxLocked = 32;
 constexpr int kBlock = 256;

@@ -259,9 +259,9 @@ int yh_state_write(const yh_reg* reg, const char* path) {
     }
     if (std::fwrite(out, 16, na, fh.f) != na) status = fail(YH_ERR_IO, "write failed");
   }
-  hipStreamSynchronize(st);
-  hipEventDestroy(ev[0]);
-  hipEventDestroy(ev[1]);
+  (void)hipStreamSynchronize(st);
+  (void)hipEventDestroy(ev[0]);
+  (void)hipEventDestroy(ev[1]);
   if (status != YH_OK) return status;
   if (std::fflush(fh.f) != 0) return fail(YH_ERR_IO, "write failed");
   return YH_OK;
@@ -315,8 +315,8 @@ int yh_state_read(const char* path, int dtype, yh_reg** out) {
   }
   if (hipStreamSynchronize(r->w.stream) != hipSuccess && status == YH_OK)
     status = fail(YH_ERR_HIP, "copy to device failed");
-  if (ev[0]) hipEventDestroy(ev[0]);
-  if (ev[1]) hipEventDestroy(ev[1]);
+  if (ev[0]) (void)hipEventDestroy(ev[0]);
+  if (ev[1]) (void)hipEventDestroy(ev[1]);
   if (status != YH_OK) {
     yh_reg_destroy(r);
     return status;
