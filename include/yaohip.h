@@ -270,7 +270,9 @@ typedef struct yh_vqe_batch yh_vqe_batch;
  * qubit, then the block's gates in application order.
  * Energy: TFIM H = -J sum Z_i Z_{i+1} - h sum X_i, open boundary.
  * dtype YH_C64 keeps amplitudes in complex64 and reduces in FP64; YH_C128 is the
- * all-FP64 path. */
+ * all-FP64 path. The first call for a given list of (nq, depth) shapes times every
+ * launch configuration on that batch and keeps the fastest for later calls with
+ * the same shapes; the launch sequence is then replayed as a captured hipGraph. */
 int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_batch **out);
 int yh_vqe_batch_destroy(yh_vqe_batch *b);
 int yh_vqe_batch_energies(yh_vqe_batch *b, int ninst, const int32_t *nq,

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <unordered_map>
 #include <vector>
 
 #include "host.h"
@@ -197,127 +198,170 @@ __host__ __device__ inline int vqe_npass(int n, int tb) {
 
 namespace {
 
+// Geometry of one work item of a step (see the comment on VqeStep).
+struct VqeItem {
+  int64_t item;  // index within the instance's step
+  int inst, n, depth, layer, ps, npass, nh, lo_bit, tbits;
+  int64_t rot_off, state_off, part_off;
+  bool gen0;
+};
+
+template <int TB>
+__device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big, const VqeStep& e, int64_t w,
+                                         VqeItem& it) {
+  it.item = w - e.item_base;
+  it.inst = e.inst;
+  const VqeInst in = insts[e.inst];
+  const VqeBig bg = big[e.inst];
+  it.n = in.n;
+  it.depth = in.depth;
+  it.rot_off = in.rot_off;
+  it.state_off = bg.state_off;
+  it.part_off = bg.part_off;
+  it.npass = vqe_npass(it.n, TB);
+  it.layer = e.step / it.npass;
+  it.ps = e.step % it.npass;
+  // tile positions 0..4 are memory bits 0..4, positions 5..tbits-1 the contiguous
+  // memory bits lo_bit..lo_bit+nh-1
+  if (it.ps == 0) {
+    it.nh = TB - kVqeLow;
+    it.lo_bit = kVqeLow;
+  } else {
+    const int cs = (it.n - TB + it.npass - 2) / (it.npass - 1);
+    it.lo_bit = TB + (it.ps - 1) * cs;
+    it.nh = min(cs, it.n - it.lo_bit);
+  }
+  it.tbits = kVqeLow + it.nh;
+  it.gen0 = it.depth == 0 || (it.layer == 0 && it.ps == 0);
+}
+
+// memory index of LDS position p: tile (item << (TB - tbits)) + (p >> tbits) fills the
+// bits outside the tile in ascending order (zeros inserted at 0..4 and at the chunk)
+template <int TB>
+__device__ __forceinline__ uint64_t vqe_mem_index(const VqeItem& it, int p) {
+  const uint64_t t = (uint64_t(it.item) << (TB - it.tbits)) + uint64_t(p >> it.tbits);
+  uint64_t y = t << kVqeLow;
+  y = ((y >> it.lo_bit) << (it.lo_bit + it.nh)) | (y & ((uint64_t(1) << it.lo_bit) - 1));
+  const uint64_t q = uint64_t(p & ((1 << it.tbits) - 1));
+  return y | (q & ((1u << kVqeLow) - 1)) | ((q >> kVqeLow) << it.lo_bit);
+}
+
+// Persistent: workgroup g takes items g, g + grid, ...; the next item's entry is
+// found by a forward scan (entries are ordered by first item) and its amplitudes
+// are loaded into registers while the current item is processed (the barriers in
+// the loop order LDS only, so those loads stay in flight).
 template <class R, int TB>
-__global__ __launch_bounds__(256) void k_vqe_step(const VqeInst* __restrict__ insts, const VqeBig* __restrict__ big,
-                                                  const VqeStep* __restrict__ ent, int nent,
-                                                  const typename Cplx<R>::V* __restrict__ rot,
-                                                  typename Cplx<R>::V* __restrict__ states,
-                                                  double* __restrict__ partials) {
+__global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__ insts,
+                                                     const VqeBig* __restrict__ big,
+                                                     const VqeStep* __restrict__ ent, int nent, int64_t nitems,
+                                                     const typename Cplx<R>::V* __restrict__ rot,
+                                                     typename Cplx<R>::V* __restrict__ states,
+                                                     double* __restrict__ partials) {
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
   constexpr int kPer = TN / 256;
   __shared__ V lds[TN + TN / 16];
   __shared__ double red[16];
   const int tid = threadIdx.x;
-  // the entry owning this workgroup (the same for every thread)
-  const int64_t w = blockIdx.x;
-  int lo = 0, hi = nent - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ent[mid].item_base <= w) lo = mid;
-    else hi = mid - 1;
-  }
-  const VqeStep e = ent[lo];
-  const uint64_t item = uint64_t(w - e.item_base);
-  const VqeInst in = insts[e.inst];
-  const VqeBig bg = big[e.inst];
-  const int n = in.n, depth = in.depth;
-  const int npass = vqe_npass(n, TB);
-  const int layer = e.step / npass, ps = e.step % npass;
-  // pass geometry: tile positions 0..4 are memory bits 0..4, positions 5..tbits-1
-  // the contiguous memory bits lo_bit..lo_bit+nh-1
-  int nh, lo_bit;
-  if (ps == 0) {
-    nh = TB - kVqeLow;
-    lo_bit = kVqeLow;
-  } else {
-    const int cs = (n - TB + npass - 2) / (npass - 1);
-    lo_bit = TB + (ps - 1) * cs;
-    nh = min(cs, n - lo_bit);
-  }
-  const int tbits = kVqeLow + nh;
-  const int tmask = (1 << tbits) - 1;
-  const uint64_t t0 = item << (TB - tbits);
-  // memory index of LDS position p: tile t0 + (p >> tbits) fills the bits outside
-  // the tile in ascending order (zeros inserted at 0..4 and at the chunk)
-  auto mem_index = [&](int p) {
-    const uint64_t t = t0 + uint64_t(p >> tbits);
-    uint64_t y = t << kVqeLow;
-    y = ((y >> lo_bit) << (lo_bit + nh)) | (y & ((uint64_t(1) << lo_bit) - 1));
-    const uint64_t q = uint64_t(p & tmask);
-    return y | (q & ((1u << kVqeLow) - 1)) | ((q >> kVqeLow) << lo_bit);
+  int64_t w = blockIdx.x;
+  if (w >= nitems) return;  // uniform
+  int ei = 0;
+  auto find = [&](int64_t ww) {
+    while (ei + 1 < nent && ent[ei + 1].item_base <= ww) ++ei;
+    return ent[ei];
   };
-  V* st = states + bg.state_off;
-  const bool gen0 = depth == 0 || (layer == 0 && ps == 0);
-  uint64_t xs[kPer];
+  VqeItem cur, nxt;
+  vqe_item<TB>(insts, big, find(w), w, cur);
+  // memory indices are recomputed where needed (a few integer operations) rather
+  // than kept in 64-bit registers for the whole item
+  V pre[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    const int p = tid + 256 * k;
-    xs[k] = mem_index(p);
-    lds[vpad<TB>(p)] = gen0 ? cmk<V>(R(xs[k] == 0 ? 1 : 0), R(0)) : st[xs[k]];
+    const uint64_t x = vqe_mem_index<TB>(cur, tid + 256 * k);
+    pre[k] = cur.gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[cur.state_off + int64_t(x)];
   }
-  lds_barrier();
-  const int rb_first = ps == 0 ? 0 : kVqeLow;
-  if (layer < depth) {
-    const V* ul = rot + in.rot_off + int64_t(layer) * n * 4;
-    const bool last_pass = ps == npass - 1;
-    for (int g0 = rb_first; g0 < tbits; g0 += 4) {
-      const int gb = min(4, tbits - g0);
-      const bool lastg = g0 + 4 >= tbits;
-      const int nsub = TN >> gb;
-      int mbk[4];
+  for (;;) {
 #pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const int pos = g0 + k;
-        const int mb = k < gb ? (pos < kVqeLow ? pos : lo_bit + pos - kVqeLow) : 0;
-        mbk[k] = __builtin_amdgcn_readfirstlane(mb);
+    for (int k = 0; k < kPer; ++k) lds[vpad<TB>(tid + 256 * k)] = pre[k];
+    lds_barrier();
+    // next item: geometry, addresses and the loads, in flight during this item
+    const int64_t wn = w + gridDim.x;
+    const bool more = wn < nitems;
+    if (more) {
+      vqe_item<TB>(insts, big, find(wn), wn, nxt);
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const uint64_t x = vqe_mem_index<TB>(nxt, tid + 256 * k);
+        pre[k] = nxt.gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[nxt.state_off + int64_t(x)];
       }
-      for (int sc = tid; sc < nsub; sc += 256) {
-        const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-        V a[16];
+    }
+    V* st = states + cur.state_off;
+    const int rb_first = cur.ps == 0 ? 0 : kVqeLow;
+    if (cur.layer < cur.depth) {
+      const V* ul = rot + cur.rot_off + int64_t(cur.layer) * cur.n * 4;
+      const bool last_pass = cur.ps == cur.npass - 1;
+      for (int g0 = rb_first; g0 < cur.tbits; g0 += 4) {
+        const int gb = min(4, cur.tbits - g0);
+        const bool lastg = g0 + 4 >= cur.tbits;
+        const int nsub = TN >> gb;
+        int mbk[4];
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+        for (int k = 0; k < 4; ++k) {
+          const int pos = g0 + k;
+          const int mb = k < gb ? (pos < kVqeLow ? pos : cur.lo_bit + pos - kVqeLow) : 0;
+          mbk[k] = __builtin_amdgcn_readfirstlane(mb);
+        }
+        for (int sc = tid; sc < nsub; sc += 256) {
+          const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+          V a[16];
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-          if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
+          for (int i = 0; i < 16; ++i)
+            if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
 #pragma unroll
-        for (int i = 0; i < 16; ++i)
-          if (i < (1 << gb)) {
-            const int p = base + (i << g0);
-            V v = a[i];
-            if (last_pass && lastg && cz_sign(mem_index(p), n) < 0) {
-              v.x = -v.x;
-              v.y = -v.y;
+          for (int k = 0; k < 4; ++k)
+            if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
+#pragma unroll
+          for (int i = 0; i < 16; ++i)
+            if (i < (1 << gb)) {
+              const int p = base + (i << g0);
+              V v = a[i];
+              if (last_pass && lastg && cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
+                v.x = -v.x;
+                v.y = -v.y;
+              }
+              lds[vpad<TB>(p)] = v;
             }
-            lds[vpad<TB>(p)] = v;
-          }
+        }
+        lds_barrier();
       }
-      lds_barrier();
-    }
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) st[xs[k]] = lds[vpad<TB>(tid + 256 * k)];
-  } else {
-    // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
-    double zz_acc = 0.0, x_acc = 0.0;
+      for (int k = 0; k < kPer; ++k) st[vqe_mem_index<TB>(cur, tid + 256 * k)] = lds[vpad<TB>(tid + 256 * k)];
+    } else {
+      // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
+      double zz_acc = 0.0, x_acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int p = tid + 256 * k;
-      const V v = lds[vpad<TB>(p)];
-      const double pr = double(v.x), pi = double(v.y);
-      if (ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(xs[k], n);
-      for (int pos = rb_first; pos < tbits; ++pos) {
-        const V q = lds[vpad<TB>(p ^ (1 << pos))];
-        x_acc += pr * double(q.x) + pi * double(q.y);
+      for (int k = 0; k < kPer; ++k) {
+        const int p = tid + 256 * k;
+        const V v = lds[vpad<TB>(p)];
+        const double pr = double(v.x), pi = double(v.y);
+        if (cur.ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(vqe_mem_index<TB>(cur, p), cur.n);
+        for (int pos = rb_first; pos < cur.tbits; ++pos) {
+          const V q = lds[vpad<TB>(p ^ (1 << pos))];
+          x_acc += pr * double(q.x) + pi * double(q.y);
+        }
+      }
+      const double zz = bred(zz_acc, red);
+      const double xx = bred(x_acc, red);
+      if (tid == 0) {
+        double* o = partials + 2 * (cur.part_off + int64_t(cur.ps) * (int64_t(1) << (cur.n - TB)) + cur.item);
+        o[0] = zz;
+        o[1] = xx;
       }
     }
-    const double zz = bred(zz_acc, red);
-    const double xx = bred(x_acc, red);
-    if (tid == 0) {
-      double* o = partials + 2 * (bg.part_off + int64_t(ps) * (int64_t(1) << (n - TB)) + int64_t(item));
-      o[0] = zz;
-      o[1] = xx;
-    }
+    lds_barrier();  // LDS is refilled for the next item
+    if (!more) break;
+    w = wn;
+    cur = nxt;
   }
 }
 
@@ -376,6 +420,9 @@ static void compose_block(const double* a, int apq, cd u[4]) {
 struct VqeConfig {
   int tile_bits = 12;
   int blocks_per_cu = 2;
+  // step launches: true = as many workgroups as stay resident, each walking many
+  // work items with the next one prefetched; false = one workgroup per work item
+  bool resident_grid = true;
 };
 
 }  // namespace yh
@@ -388,7 +435,8 @@ struct yh_vqe_batch {
   Workspace w;
   DevBuf insts, rot, energies, counter, big, steps, states, partials;
   VqeConfig cfg;
-  bool tuned = false;
+  // autotuned configuration per instance-shape hash (the batch's n and depth list)
+  std::unordered_map<uint64_t, VqeConfig> tuned;
   std::vector<int> sorted_n, sorted_d;  // instance shapes in upload order (n descending)
   // step plan of the HBM-resident instances for one tile size
   struct Launch {
@@ -479,10 +527,16 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   if (b->nbig < ninst)
     hipLaunchKernelGGL((k_vqe<R, TB>), dim3(b->cus * cfg.blocks_per_cu), dim3(256), 0, b->w.stream, ins, b->nbig,
                        ninst, ctr + 1, rot, en, J, h);
+  static int per_cu = 0;  // one per (R, TB) instantiation
+  if (per_cu == 0) {
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vqe_step<R, TB>, 256, 0));
+    per_cu = std::max(1, per_cu);
+  }
+  const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
   for (const auto& l : b->launches)
-    hipLaunchKernelGGL((k_vqe_step<R, TB>), dim3(unsigned(l.items)), dim3(256), 0, b->w.stream, ins, big,
-                       ent + l.entry_off, int(l.nentries), rot, static_cast<V*>(b->states.p),
-                       static_cast<double*>(b->partials.p));
+    hipLaunchKernelGGL((k_vqe_step<R, TB>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(256), 0,
+                       b->w.stream, ins, big, ent + l.entry_off, int(l.nentries), l.items, rot,
+                       static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
   if (b->nbig > 0)
     hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
                        static_cast<const double*>(b->partials.p), en, J, h);
@@ -604,11 +658,15 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   if (!b || !nq || !depth || !offset || !angles || !energies) return fail(YH_ERR_INVALID_ARG, "null argument");
   if (ninst < 1 || ninst > b->max_instances) return fail(YH_ERR_INVALID_ARG, "instance count out of range");
   YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles, apq));
-  if (!b->tuned) {
+  const uint64_t shape = vqe_shape_key(b, ninst, 0);
+  auto hit = b->tuned.find(shape);
+  if (hit != b->tuned.end()) {
+    b->cfg = hit->second;
+  } else {
     // autotune: time every candidate (tile bits, LDS-kernel workgroups per CU) on
     // this batch, keep the fastest
     double best = 1e30;
-    for (const VqeConfig& c : vqe_candidates()) {
+    auto trial = [&](const VqeConfig& c) -> int {
       YH_TRY(vqe_prepare(b, ninst, c.tile_bits));
       YH_TRY(vqe_run(b, ninst, J, h, c));  // warm
       YH_TRY(b->w.sync());
@@ -620,13 +678,21 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
         best = dt;
         b->cfg = c;
       }
+      return YH_OK;
+    };
+    for (const VqeConfig& c : vqe_candidates()) YH_TRY(trial(c));
+    {  // then the step-launch grid, on the winning tile size
+      VqeConfig c = b->cfg;
+      c.resident_grid = !c.resident_grid;
+      YH_TRY(trial(c));
     }
-    b->tuned = true;
+    if (b->tuned.size() >= 64) b->tuned.clear();
+    b->tuned.emplace(shape, b->cfg);
   }
   YH_TRY(vqe_prepare(b, ninst, b->cfg.tile_bits));
   // replay a captured graph of the launch sequence; the key covers the instance
   // shapes, the configuration and every device buffer the launches read or write
-  uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56);
+  uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56) ^ (uint64_t(b->cfg.resident_grid) << 62);
   for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->states.p,
                         b->partials.p})
     key = (key ^ uint64_t(reinterpret_cast<uintptr_t>(p))) * 1099511628211ull;
