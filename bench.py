@@ -166,6 +166,19 @@ def main():
     roof = roofline_of(kstats)
     if n == 28:
         roof["traffic"] = BLOCK28_TRAFFIC_BYTES
+    # practical ceiling beside the published peak: a device-to-device copy of the
+    # same state (read + write every amplitude once), fastest of 5
+    other = yaohip.ArrayReg.zero_state(n)
+    copy_s = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        A.check(L.yh_reg_copy(other._h, reg._h))  # synchronizes before returning
+        copy_s.append(time.perf_counter() - t0)
+    del other
+    copy_gbps = 2 * 16 * (1 << n) / min(copy_s[1:]) / 1e9
+    roof["copy_gbps"] = round(copy_gbps, 1)
+    roof["frac_of_copy"] = round(roof["achieved"] / copy_gbps, 4)
+    log(f"device copy of the state: {copy_gbps:.1f} GB/s; dominant kernel at {roof['frac_of_copy']:.3f} of it")
     out = {
         "metric": "gate-apps/s at 28q c128 (random brickwork depth 20)",
         "value": round(value, 2), "unit": "gate-apps/s", "n_gpus": 1, "steps": args.steps,

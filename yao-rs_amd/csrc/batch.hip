@@ -183,9 +183,18 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
 // (one workgroup) takes 2^(TB - tbits) consecutive tiles: 2^TB amplitudes, and
 // 2^(n - TB) items per instance in every step.
 struct VqeStep {
-  int32_t inst;       // sorted instance index
-  int32_t step;       // step of that instance
-  int64_t item_base;  // first work item of this entry within the launch
+  int32_t inst;  // sorted instance index
+  int32_t step;  // step of that instance
+};
+// A launch's entries are in instance order, so qubit counts do not increase along
+// them; a segment is a run of entries with one qubit count, whose items are
+// 2^shift per entry. Work item w of the launch, in the segment with
+// item_base <= w, is item (w - item_base) mod 2^shift of entry
+// ent_base + (w - item_base) >> shift.
+struct VqeSeg {
+  int64_t item_base;
+  int32_t ent_base;
+  int32_t shift;
 };
 struct VqeBig {
   int64_t state_off;  // amplitude offset of the instance's state
@@ -207,9 +216,9 @@ struct VqeItem {
 };
 
 template <int TB>
-__device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big, const VqeStep& e, int64_t w,
+__device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big, const VqeStep& e, int64_t item,
                                          VqeItem& it) {
-  it.item = w - e.item_base;
+  it.item = item;
   it.inst = e.inst;
   const VqeInst in = insts[e.inst];
   const VqeBig bg = big[e.inst];
@@ -246,14 +255,15 @@ __device__ __forceinline__ uint64_t vqe_mem_index(const VqeItem& it, int p) {
   return y | (q & ((1u << kVqeLow) - 1)) | ((q >> kVqeLow) << it.lo_bit);
 }
 
-// Persistent: workgroup g takes items g, g + grid, ...; the next item's entry is
-// found by a forward scan (entries are ordered by first item) and its amplitudes
-// are loaded into registers while the current item is processed (the barriers in
-// the loop order LDS only, so those loads stay in flight).
+// Persistent: workgroup g takes items g, g + grid, ...; the next item's segment is
+// found by a forward scan over the launch's few segments, its entry by one shift,
+// and its amplitudes are loaded into registers while the current item is processed
+// (the barriers in the loop order LDS only, so those loads stay in flight).
 template <class R, int TB>
 __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__ insts,
                                                      const VqeBig* __restrict__ big,
-                                                     const VqeStep* __restrict__ ent, int nent, int64_t nitems,
+                                                     const VqeStep* __restrict__ ent,
+                                                     const VqeSeg* __restrict__ seg, int nseg, int64_t nitems,
                                                      const typename Cplx<R>::V* __restrict__ rot,
                                                      typename Cplx<R>::V* __restrict__ states,
                                                      double* __restrict__ partials) {
@@ -265,13 +275,15 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
   const int tid = threadIdx.x;
   int64_t w = blockIdx.x;
   if (w >= nitems) return;  // uniform
-  int ei = 0;
-  auto find = [&](int64_t ww) {
-    while (ei + 1 < nent && ent[ei + 1].item_base <= ww) ++ei;
-    return ent[ei];
+  int si = 0;
+  auto locate = [&](int64_t ww, VqeItem& it) {
+    while (si + 1 < nseg && seg[si + 1].item_base <= ww) ++si;
+    const VqeSeg sg = seg[si];
+    const int64_t rel = ww - sg.item_base;
+    vqe_item<TB>(insts, big, ent[sg.ent_base + (rel >> sg.shift)], rel & ((int64_t(1) << sg.shift) - 1), it);
   };
   VqeItem cur, nxt;
-  vqe_item<TB>(insts, big, find(w), w, cur);
+  locate(w, cur);
   // memory indices are recomputed where needed (a few integer operations) rather
   // than kept in 64-bit registers for the whole item
   V pre[kPer];
@@ -288,7 +300,7 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
     const int64_t wn = w + gridDim.x;
     const bool more = wn < nitems;
     if (more) {
-      vqe_item<TB>(insts, big, find(wn), wn, nxt);
+      locate(wn, nxt);
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const uint64_t x = vqe_mem_index<TB>(nxt, tid + 256 * k);
@@ -433,14 +445,14 @@ struct yh_vqe_batch {
   int max_instances = 0, max_qubits = 0, dtype = YH_C64;
   int cus = 256;
   Workspace w;
-  DevBuf insts, rot, energies, counter, big, steps, states, partials;
+  DevBuf insts, rot, energies, counter, big, steps, segs, states, partials;
   VqeConfig cfg;
   // autotuned configuration per instance-shape hash (the batch's n and depth list)
   std::unordered_map<uint64_t, VqeConfig> tuned;
   std::vector<int> sorted_n, sorted_d;  // instance shapes in upload order (n descending)
   // step plan of the HBM-resident instances for one tile size
   struct Launch {
-    int64_t entry_off, nentries, items;
+    int64_t entry_off, seg_off, nseg, items;
   };
   int plan_tb = -1;
   uint64_t plan_key = 0;
@@ -488,15 +500,20 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
     smax = std::max(smax, (b->sorted_d[i] + 1) * vqe_npass(n, tb));
   }
   std::vector<VqeStep> ent;
+  std::vector<VqeSeg> seg;
   std::vector<yh_vqe_batch::Launch> L;
   for (int k = 0; k < smax; ++k) {
-    yh_vqe_batch::Launch l{int64_t(ent.size()), 0, 0};
+    const int64_t e0 = int64_t(ent.size());
+    yh_vqe_batch::Launch l{e0, int64_t(seg.size()), 0, 0};
     for (int i = 0; i < nbig; ++i) {
       const int n = b->sorted_n[i];
       if (k < (b->sorted_d[i] + 1) * vqe_npass(n, tb)) {
-        ent.push_back(VqeStep{i, k, l.items});
+        if (l.nseg == 0 || seg.back().shift != n - tb) {
+          seg.push_back(VqeSeg{l.items, int32_t(int64_t(ent.size()) - e0), n - tb});
+          ++l.nseg;
+        }
+        ent.push_back(VqeStep{i, k});
         l.items += int64_t(1) << (n - tb);
-        ++l.nentries;
       }
     }
     L.push_back(l);
@@ -506,6 +523,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
   YH_TRY(b->partials.ensure(std::max<int64_t>(poff, 1) * 2 * sizeof(double)));
   YH_TRY(b->w.upload(big.data(), big.size() * sizeof(VqeBig), b->big));
   if (!ent.empty()) YH_TRY(b->w.upload(ent.data(), ent.size() * sizeof(VqeStep), b->steps));
+  if (!seg.empty()) YH_TRY(b->w.upload(seg.data(), seg.size() * sizeof(VqeSeg), b->segs));
   YH_TRY(b->w.sync());
   b->nbig = nbig;
   b->launches = L;
@@ -523,6 +541,7 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   double* en = static_cast<double*>(b->energies.p);
   const VqeBig* big = static_cast<const VqeBig*>(b->big.p);
   const VqeStep* ent = static_cast<const VqeStep*>(b->steps.p);
+  const VqeSeg* sg = static_cast<const VqeSeg*>(b->segs.p);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, ctr);
   if (b->nbig < ninst)
     hipLaunchKernelGGL((k_vqe<R, TB>), dim3(b->cus * cfg.blocks_per_cu), dim3(256), 0, b->w.stream, ins, b->nbig,
@@ -535,7 +554,7 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
   for (const auto& l : b->launches)
     hipLaunchKernelGGL((k_vqe_step<R, TB>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(256), 0,
-                       b->w.stream, ins, big, ent + l.entry_off, int(l.nentries), l.items, rot,
+                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, rot,
                        static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
   if (b->nbig > 0)
     hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
@@ -693,8 +712,8 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   // replay a captured graph of the launch sequence; the key covers the instance
   // shapes, the configuration and every device buffer the launches read or write
   uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56) ^ (uint64_t(b->cfg.resident_grid) << 62);
-  for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->states.p,
-                        b->partials.p})
+  for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->segs.p,
+                        b->states.p, b->partials.p})
     key = (key ^ uint64_t(reinterpret_cast<uintptr_t>(p))) * 1099511628211ull;
   if (!b->graph || b->graph_key != key) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
