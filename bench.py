@@ -30,11 +30,11 @@ HBM_PEAK_GBPS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 # HBM bytes per launch of k_block5<double> on the 28-qubit state, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 72 launches
+# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 48 launches
 # of one step), each scaled by the factor a plain copy of known size gives for the
 # same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
 # profiles/k_block5_brickwork28_pmc.md.
-BLOCK28_TRAFFIC_BYTES = int((2 * 2101040 + 4244230) * 1024)
+BLOCK28_TRAFFIC_BYTES = int((2 * 2097825 + 4256224) * 1024)
 NBITS = 28
 DEPTH = 20
 SEED = 20

@@ -275,6 +275,17 @@ typedef struct yh_vqe_batch yh_vqe_batch;
  * the same shapes; the launch sequence is then replayed as a captured hipGraph. */
 int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_batch **out);
 int yh_vqe_batch_destroy(yh_vqe_batch *b);
+/* Fix the launch configuration instead of autotuning: tile_bits in {11, 12, 13}
+ * (instances with more qubits stream through HBM in 2^tile_bits-amplitude work
+ * items), blocks_per_cu workgroups per CU for the LDS-resident kernel,
+ * resident_grid != 0 for step launches of as many workgroups as stay resident
+ * (else one workgroup per work item), step_threads 256 or 512 (workgroup size of
+ * the streaming kernel). tile_bits = 0 returns to autotuning. */
+int yh_vqe_batch_configure(yh_vqe_batch *b, int tile_bits, int blocks_per_cu, int resident_grid,
+                           int step_threads);
+/* The configuration of the last yh_vqe_batch_energies call: tile_bits,
+ * blocks_per_cu, resident_grid, step_threads. */
+int yh_vqe_batch_config(const yh_vqe_batch *b, int32_t out[4]);
 int yh_vqe_batch_energies(yh_vqe_batch *b, int ninst, const int32_t *nq,
                           const int32_t *depth, const int64_t *offset, const double *angles,
                           int angles_per_qubit, double J, double h, double *energies);

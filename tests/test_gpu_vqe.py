@@ -50,6 +50,26 @@ def test_vqe_batch_vs_plain_path_large_instances():
     assert np.max(np.abs(got - plain)) < 1e-10
 
 
+@pytest.mark.parametrize("tile_bits", [11, 12, 13])
+@pytest.mark.parametrize("resident,threads", [(True, 256), (False, 256), (True, 512)])
+def test_vqe_batch_every_configuration_vs_oracle(tile_bits, resident, threads):
+    """Each fixed launch configuration against the oracle: instances above the tile
+    size stream through HBM in 2 chunks, and at tile_bits 11 and 12 in 3 chunks
+    (passes that visit the chunks back and forth), depth 0 included."""
+    inst = random_instances(16, seed=0, nmin=tile_bits - 2, nmax=min(20, tile_bits + 8), dmin=0, dmax=5)
+    if tile_bits < 13:
+        assert inst.nq.max() >= 2 * tile_bits - 4  # a three-chunk instance is present
+    want = oracle_energies(inst)
+    b = VqeBatch(32, 20, "c128").configure(tile_bits, 2, resident, threads)
+    got = b.energies(inst)
+    assert b.config() == {"tile_bits": tile_bits, "blocks_per_cu": 2, "resident_grid": resident,
+                          "step_threads": threads}
+    err = np.max(np.abs(got - want))
+    print(f"tile_bits={tile_bits} resident={resident} threads={threads}: max |dE| vs oracle {err:.3e}")
+    assert err < 1e-10
+    assert np.array_equal(b.energies(inst), got)  # graph replay
+
+
 def test_vqe_batch_repeat_and_coefficients():
     """Graph replays give the same energies; J and h are honoured."""
     inst = random_instances(30, seed=10, nmin=8, nmax=18, dmin=1, dmax=5)

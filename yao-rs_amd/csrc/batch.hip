@@ -12,9 +12,11 @@
 // Instances with at most 2^TB amplitudes live in LDS for their whole evaluation:
 // one persistent kernel whose workgroups pull them from an atomic queue. Larger
 // instances keep their states in HBM and advance one step per launch (a step is
-// one tile pass of one layer), every (instance, 2^TB-amplitude work item) being its
-// own workgroup, so the whole GPU streams all large states at once; a layer costs
-// 1 + ceil((n - TB) / (TB - 5)) passes.
+// one tile pass), every (instance, 2^TB-amplitude work item) being its own
+// workgroup, so the whole GPU streams all large states at once. Their bits are
+// split into k = 1 + ceil((n - TB) / (TB - 5)) chunks that the passes visit back and
+// forth; the pass that finishes a layer also applies its CZ chain and starts the
+// next layer on the bits it holds, so a layer costs k - 1 passes, not k.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -43,21 +45,6 @@ constexpr int kVqeLow = 5;
 template <int TB>
 __device__ __forceinline__ int vpad(int p) {
   return p + (p >> 4);
-}
-
-// streaming (non-temporal) access to the per-workgroup HBM slot
-template <class V>
-__device__ __forceinline__ V nt_load(const V* p) {
-  using R = decltype(V::x);
-  const R* q = reinterpret_cast<const R*>(p);
-  return cmk<V>(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
-}
-template <class V>
-__device__ __forceinline__ void nt_store(V v, V* p) {
-  using R = decltype(V::x);
-  R* q = reinterpret_cast<R*>(p);
-  __builtin_nontemporal_store(R(v.x), q);
-  __builtin_nontemporal_store(R(v.y), q + 1);
 }
 
 __device__ __forceinline__ double cz_sign(uint64_t x, int n) {
@@ -176,12 +163,20 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
 }  // namespace
 
 // Instances with n > TB: their states live in HBM (one buffer each) and every
-// instance advances one step per launch. Step k of an instance is pass
-// k % npass of layer k / npass (layer == depth: the energy passes). Pass 0 holds
-// memory bits 0..TB-1; pass c > 0 holds bits 0..4 plus the c-th of npass - 1 equal
-// chunks of bits TB..n-1, so a pass's tile has tbits <= TB bits and a work item
-// (one workgroup) takes 2^(TB - tbits) consecutive tiles: 2^TB amplitudes, and
-// 2^(n - TB) items per instance in every step.
+// instance advances one step per launch. The memory bits outside 0..4 form
+// k = npass chunks: chunk 0 is bits 5..TB-1, chunk c > 0 the c-th of k - 1 equal
+// parts of bits TB..n-1. The tile of a step holds bits 0..4 plus one chunk, so it
+// has tbits <= TB bits and a work item (one workgroup) takes 2^(TB - tbits)
+// consecutive tiles: 2^TB amplitudes, and 2^(n - TB) items per instance in every
+// step. Step j holds chunk bounce(j): 0, 1, .., k-1, k-2, .., 0, 1, ..; with
+// T = depth (k - 1):
+//   j = 0: layer 0 on every bit of the tile (the state starts as |0...0>);
+//   0 < j <= T, j a multiple of k - 1 (an end chunk): layer L = j / (k - 1) - 1 on
+//     the chunk, the CZ chain (layer L is then complete), then layer L + 1 on every
+//     bit of the tile if L + 1 < depth;
+//   other j <= T: layer j / (k - 1) on the chunk;
+//   j >= T: energy terms: X on the chunk's bits, plus ZZ and X on bits 0..4 at j = T.
+// Steps j <= T write the tile back; an instance takes T + k steps.
 struct VqeStep {
   int32_t inst;  // sorted instance index
   int32_t step;  // step of that instance
@@ -204,16 +199,27 @@ struct VqeBig {
 __host__ __device__ inline int vqe_npass(int n, int tb) {
   return n <= tb ? 1 : 1 + (n - tb + (tb - kVqeLow) - 1) / (tb - kVqeLow);
 }
+// steps of an instance with n > tb: depth (k - 1) rotation steps, then k energy
+// steps, the first of which also completes the last layer
+inline int vqe_steps(int n, int depth, int tb) {
+  const int k = vqe_npass(n, tb);
+  return depth * (k - 1) + k;
+}
 
 namespace {
 
 // Geometry of one work item of a step (see the comment on VqeStep).
 struct VqeItem {
   int64_t item;  // index within the instance's step
-  int inst, n, depth, layer, ps, npass, nh, lo_bit, tbits;
+  int inst, n, depth, step, last, chunk, npass, nh, lo_bit, tbits;
   int64_t rot_off, state_off, part_off;
-  bool gen0;
 };
+
+// chunk held by step j of k chunks (k >= 2): 0, 1, .., k-1, k-2, .., 1, 0, 1, ..
+__host__ __device__ inline int vqe_bounce(int j, int k) {
+  const int r = j % (2 * (k - 1));
+  return r < k ? r : 2 * (k - 1) - r;
+}
 
 template <int TB>
 __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big, const VqeStep& e, int64_t item,
@@ -228,20 +234,20 @@ __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big
   it.state_off = bg.state_off;
   it.part_off = bg.part_off;
   it.npass = vqe_npass(it.n, TB);
-  it.layer = e.step / it.npass;
-  it.ps = e.step % it.npass;
+  it.step = e.step;
+  it.last = it.depth * (it.npass - 1);
+  it.chunk = vqe_bounce(e.step, it.npass);
   // tile positions 0..4 are memory bits 0..4, positions 5..tbits-1 the contiguous
   // memory bits lo_bit..lo_bit+nh-1
-  if (it.ps == 0) {
+  if (it.chunk == 0) {
     it.nh = TB - kVqeLow;
     it.lo_bit = kVqeLow;
   } else {
     const int cs = (it.n - TB + it.npass - 2) / (it.npass - 1);
-    it.lo_bit = TB + (it.ps - 1) * cs;
+    it.lo_bit = TB + (it.chunk - 1) * cs;
     it.nh = min(cs, it.n - it.lo_bit);
   }
   it.tbits = kVqeLow + it.nh;
-  it.gen0 = it.depth == 0 || (it.layer == 0 && it.ps == 0);
 }
 
 // memory index of LDS position p: tile (item << (TB - tbits)) + (p >> tbits) fills the
@@ -259,8 +265,8 @@ __device__ __forceinline__ uint64_t vqe_mem_index(const VqeItem& it, int p) {
 // found by a forward scan over the launch's few segments, its entry by one shift,
 // and its amplitudes are loaded into registers while the current item is processed
 // (the barriers in the loop order LDS only, so those loads stay in flight).
-template <class R, int TB>
-__global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__ insts,
+template <class R, int TB, int NT>
+__global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ insts,
                                                      const VqeBig* __restrict__ big,
                                                      const VqeStep* __restrict__ ent,
                                                      const VqeSeg* __restrict__ seg, int nseg, int64_t nitems,
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
                                                      double* __restrict__ partials) {
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
-  constexpr int kPer = TN / 256;
+  constexpr int kPer = TN / NT;
   __shared__ V lds[TN + TN / 16];
   __shared__ double red[16];
   const int tid = threadIdx.x;
@@ -289,12 +295,12 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
   V pre[kPer];
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
-    const uint64_t x = vqe_mem_index<TB>(cur, tid + 256 * k);
-    pre[k] = cur.gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[cur.state_off + int64_t(x)];
+    const uint64_t x = vqe_mem_index<TB>(cur, tid + NT * k);
+    pre[k] = cur.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[cur.state_off + int64_t(x)];
   }
   for (;;) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) lds[vpad<TB>(tid + 256 * k)] = pre[k];
+    for (int k = 0; k < kPer; ++k) lds[vpad<TB>(tid + NT * k)] = pre[k];
     lds_barrier();
     // next item: geometry, addresses and the loads, in flight during this item
     const int64_t wn = w + gridDim.x;
@@ -303,18 +309,17 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
       locate(wn, nxt);
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
-        const uint64_t x = vqe_mem_index<TB>(nxt, tid + 256 * k);
-        pre[k] = nxt.gen0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[nxt.state_off + int64_t(x)];
+        const uint64_t x = vqe_mem_index<TB>(nxt, tid + NT * k);
+        pre[k] = nxt.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[nxt.state_off + int64_t(x)];
       }
     }
-    V* st = states + cur.state_off;
-    const int rb_first = cur.ps == 0 ? 0 : kVqeLow;
-    if (cur.layer < cur.depth) {
-      const V* ul = rot + cur.rot_off + int64_t(cur.layer) * cur.n * 4;
-      const bool last_pass = cur.ps == cur.npass - 1;
-      for (int g0 = rb_first; g0 < cur.tbits; g0 += 4) {
+    // layer `layer` on tile positions first..tbits-1, in groups of 4 bits held in
+    // registers; with `cz` the CZ chain's sign is applied as the last group is stored
+    auto rotate = [&](int layer, int first, bool cz) {
+      const V* ul = rot + cur.rot_off + int64_t(layer) * cur.n * 4;
+      for (int g0 = first; g0 < cur.tbits; g0 += 4) {
         const int gb = min(4, cur.tbits - g0);
-        const bool lastg = g0 + 4 >= cur.tbits;
+        const bool signs = cz && g0 + 4 >= cur.tbits;
         const int nsub = TN >> gb;
         int mbk[4];
 #pragma unroll
@@ -323,7 +328,7 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
           const int mb = k < gb ? (pos < kVqeLow ? pos : cur.lo_bit + pos - kVqeLow) : 0;
           mbk[k] = __builtin_amdgcn_readfirstlane(mb);
         }
-        for (int sc = tid; sc < nsub; sc += 256) {
+        for (int sc = tid; sc < nsub; sc += NT) {
           const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
           V a[16];
 #pragma unroll
@@ -337,7 +342,7 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
             if (i < (1 << gb)) {
               const int p = base + (i << g0);
               V v = a[i];
-              if (last_pass && lastg && cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
+              if (signs && cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
                 v.x = -v.x;
                 v.y = -v.y;
               }
@@ -346,18 +351,33 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
         }
         lds_barrier();
       }
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) st[vqe_mem_index<TB>(cur, tid + 256 * k)] = lds[vpad<TB>(tid + 256 * k)];
-    } else {
-      // energy: ZZ in pass 0 for every amplitude; X for the bits this pass owns
+    };
+    const int j = cur.step, k1 = cur.npass - 1;
+    if (j == 0) {
+      if (cur.depth > 0) rotate(0, 0, false);
+    } else if (j <= cur.last) {
+      if (j % k1 == 0) {
+        const int done = j / k1 - 1;
+        rotate(done, kVqeLow, true);
+        if (done + 1 < cur.depth) rotate(done + 1, 0, false);
+      } else {
+        rotate(j / k1, kVqeLow, false);
+      }
+    }
+    if (j >= cur.last) {
+      // energy terms of the final state: ZZ (every amplitude) and X on bits 0..4 in
+      // step `last`, X on the chunk's bits in every energy step; one partial pair
+      // per (chunk, item)
+      const bool first_e = j == cur.last;
+      const int xfirst = first_e ? 0 : kVqeLow;
       double zz_acc = 0.0, x_acc = 0.0;
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
-        const int p = tid + 256 * k;
+        const int p = tid + NT * k;
         const V v = lds[vpad<TB>(p)];
         const double pr = double(v.x), pi = double(v.y);
-        if (cur.ps == 0) zz_acc += (pr * pr + pi * pi) * zz_of(vqe_mem_index<TB>(cur, p), cur.n);
-        for (int pos = rb_first; pos < cur.tbits; ++pos) {
+        if (first_e) zz_acc += (pr * pr + pi * pi) * zz_of(vqe_mem_index<TB>(cur, p), cur.n);
+        for (int pos = xfirst; pos < cur.tbits; ++pos) {
           const V q = lds[vpad<TB>(p ^ (1 << pos))];
           x_acc += pr * double(q.x) + pi * double(q.y);
         }
@@ -365,10 +385,15 @@ __global__ __launch_bounds__(256, 2) void k_vqe_step(const VqeInst* __restrict__
       const double zz = bred(zz_acc, red);
       const double xx = bred(x_acc, red);
       if (tid == 0) {
-        double* o = partials + 2 * (cur.part_off + int64_t(cur.ps) * (int64_t(1) << (cur.n - TB)) + cur.item);
+        double* o = partials + 2 * (cur.part_off + int64_t(cur.chunk) * (int64_t(1) << (cur.n - TB)) + cur.item);
         o[0] = zz;
         o[1] = xx;
       }
+    }
+    if (j <= cur.last) {
+      V* st = states + cur.state_off;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) st[vqe_mem_index<TB>(cur, tid + NT * k)] = lds[vpad<TB>(tid + NT * k)];
     }
     lds_barrier();  // LDS is refilled for the next item
     if (!more) break;
@@ -432,6 +457,7 @@ static void compose_block(const double* a, int apq, cd u[4]) {
 struct VqeConfig {
   int tile_bits = 12;
   int blocks_per_cu = 2;
+  int step_threads = 256;  // workgroup size of the step kernel (256 or 512)
   // step launches: true = as many workgroups as stay resident, each walking many
   // work items with the next one prefetched; false = one workgroup per work item
   bool resident_grid = true;
@@ -449,6 +475,8 @@ struct yh_vqe_batch {
   VqeConfig cfg;
   // autotuned configuration per instance-shape hash (the batch's n and depth list)
   std::unordered_map<uint64_t, VqeConfig> tuned;
+  bool pinned = false;  // yh_vqe_batch_configure fixed the configuration
+  VqeConfig pinned_cfg;
   std::vector<int> sorted_n, sorted_d;  // instance shapes in upload order (n descending)
   // step plan of the HBM-resident instances for one tile size
   struct Launch {
@@ -497,7 +525,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
     big[i].part_off = poff;
     soff += int64_t(1) << n;
     poff += int64_t(vqe_npass(n, tb)) << (n - tb);
-    smax = std::max(smax, (b->sorted_d[i] + 1) * vqe_npass(n, tb));
+    smax = std::max(smax, vqe_steps(n, b->sorted_d[i], tb));
   }
   std::vector<VqeStep> ent;
   std::vector<VqeSeg> seg;
@@ -507,7 +535,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
     yh_vqe_batch::Launch l{e0, int64_t(seg.size()), 0, 0};
     for (int i = 0; i < nbig; ++i) {
       const int n = b->sorted_n[i];
-      if (k < (b->sorted_d[i] + 1) * vqe_npass(n, tb)) {
+      if (k < vqe_steps(n, b->sorted_d[i], tb)) {
         if (l.nseg == 0 || seg.back().shift != n - tb) {
           seg.push_back(VqeSeg{l.items, int32_t(int64_t(ent.size()) - e0), n - tb});
           ++l.nseg;
@@ -532,6 +560,23 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
   return YH_OK;
 }
 
+template <class R, int TB, int NT>
+static int vqe_steps_launch(yh_vqe_batch* b, const VqeConfig& cfg, const VqeInst* ins, const VqeBig* big,
+                            const VqeStep* ent, const VqeSeg* sg, const typename Cplx<R>::V* rot) {
+  using V = typename Cplx<R>::V;
+  static int per_cu = 0;  // one per (R, TB, NT) instantiation
+  if (per_cu == 0) {
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vqe_step<R, TB, NT>, NT, 0));
+    per_cu = std::max(1, per_cu);
+  }
+  const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
+  for (const auto& l : b->launches)
+    hipLaunchKernelGGL((k_vqe_step<R, TB, NT>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(NT), 0,
+                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, rot,
+                       static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
+  return YH_OK;
+}
+
 template <class R, int TB>
 static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const VqeConfig& cfg) {
   using V = typename Cplx<R>::V;
@@ -546,16 +591,10 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   if (b->nbig < ninst)
     hipLaunchKernelGGL((k_vqe<R, TB>), dim3(b->cus * cfg.blocks_per_cu), dim3(256), 0, b->w.stream, ins, b->nbig,
                        ninst, ctr + 1, rot, en, J, h);
-  static int per_cu = 0;  // one per (R, TB) instantiation
-  if (per_cu == 0) {
-    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vqe_step<R, TB>, 256, 0));
-    per_cu = std::max(1, per_cu);
-  }
-  const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
-  for (const auto& l : b->launches)
-    hipLaunchKernelGGL((k_vqe_step<R, TB>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(256), 0,
-                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, rot,
-                       static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
+  if (cfg.step_threads == 512)
+    YH_TRY((vqe_steps_launch<R, TB, 512>(b, cfg, ins, big, ent, sg, rot)));
+  else
+    YH_TRY((vqe_steps_launch<R, TB, 256>(b, cfg, ins, big, ent, sg, rot)));
   if (b->nbig > 0)
     hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
                        static_cast<const double*>(b->partials.p), en, J, h);
@@ -610,6 +649,32 @@ int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_bat
     return st;
   }
   *out = b;
+  return YH_OK;
+}
+
+int yh_vqe_batch_configure(yh_vqe_batch* b, int tile_bits, int blocks_per_cu, int resident_grid, int step_threads) {
+  if (!b) return fail(YH_ERR_INVALID_ARG, "null batch");
+  if (tile_bits == 0) {
+    b->pinned = false;
+    return YH_OK;
+  }
+  if (tile_bits < 11 || tile_bits > 13) return fail(YH_ERR_INVALID_ARG, "tile_bits must be 0, 11, 12 or 13");
+  if (blocks_per_cu < 1 || blocks_per_cu > 8) return fail(YH_ERR_INVALID_ARG, "blocks_per_cu must be in [1, 8]");
+  if (step_threads != 256 && step_threads != 512) return fail(YH_ERR_INVALID_ARG, "step_threads must be 256 or 512");
+  b->pinned_cfg.step_threads = step_threads;
+  b->pinned_cfg.tile_bits = tile_bits;
+  b->pinned_cfg.blocks_per_cu = blocks_per_cu;
+  b->pinned_cfg.resident_grid = resident_grid != 0;
+  b->pinned = true;
+  return YH_OK;
+}
+
+int yh_vqe_batch_config(const yh_vqe_batch* b, int32_t out[4]) {
+  if (!b || !out) return fail(YH_ERR_INVALID_ARG, "null argument");
+  out[0] = b->cfg.tile_bits;
+  out[1] = b->cfg.blocks_per_cu;
+  out[2] = b->cfg.resident_grid ? 1 : 0;
+  out[3] = b->cfg.step_threads;
   return YH_OK;
 }
 
@@ -679,7 +744,9 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   YH_TRY(vqe_upload(b, ninst, nq, depth, offset, angles, apq));
   const uint64_t shape = vqe_shape_key(b, ninst, 0);
   auto hit = b->tuned.find(shape);
-  if (hit != b->tuned.end()) {
+  if (b->pinned) {
+    b->cfg = b->pinned_cfg;
+  } else if (hit != b->tuned.end()) {
     b->cfg = hit->second;
   } else {
     // autotune: time every candidate (tile bits, LDS-kernel workgroups per CU) on
@@ -700,9 +767,12 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
       return YH_OK;
     };
     for (const VqeConfig& c : vqe_candidates()) YH_TRY(trial(c));
-    {  // then the step-launch grid, on the winning tile size
+    {  // then the step-launch grid and workgroup size, on the winning tile size
       VqeConfig c = b->cfg;
       c.resident_grid = !c.resident_grid;
+      YH_TRY(trial(c));
+      c = b->cfg;
+      c.step_threads = 512;
       YH_TRY(trial(c));
     }
     if (b->tuned.size() >= 64) b->tuned.clear();
@@ -711,7 +781,8 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   YH_TRY(vqe_prepare(b, ninst, b->cfg.tile_bits));
   // replay a captured graph of the launch sequence; the key covers the instance
   // shapes, the configuration and every device buffer the launches read or write
-  uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56) ^ (uint64_t(b->cfg.resident_grid) << 62);
+  uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56) ^ (uint64_t(b->cfg.resident_grid) << 62) ^
+                 (uint64_t(b->cfg.step_threads) << 40);
   for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->segs.p,
                         b->states.p, b->partials.p})
     key = (key ^ uint64_t(reinterpret_cast<uintptr_t>(p))) * 1099511628211ull;

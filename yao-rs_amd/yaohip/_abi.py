@@ -167,6 +167,8 @@ SIGNATURES = [
     ("yh_dm_timing_read", c_int, [c_void_p, c_int, P(ctypes.c_uint64), P(c_double)]),
     ("yh_vqe_batch_create", c_int, [c_int, c_int, c_int, P(c_void_p)]),
     ("yh_vqe_batch_destroy", c_int, [c_void_p]),
+    ("yh_vqe_batch_configure", c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    ("yh_vqe_batch_config", c_int, [c_void_p, P(ctypes.c_int32)]),
     ("yh_vqe_batch_energies", c_int, [c_void_p, c_int, P(ctypes.c_int32), P(ctypes.c_int32),
                                       P(ctypes.c_int64), P(c_double), c_int, c_double, c_double,
                                       P(c_double)]),

@@ -92,6 +92,19 @@ class VqeBatch:
             A._lib.yh_vqe_batch_destroy(h)
             self._h = None
 
+    def configure(self, tile_bits=0, blocks_per_cu=2, resident_grid=True, step_threads=256):
+        """Fix the launch configuration (tile_bits 11, 12 or 13); tile_bits=0 autotunes."""
+        A.check(A.lib().yh_vqe_batch_configure(self._h, tile_bits, blocks_per_cu, int(bool(resident_grid)),
+                                               step_threads))
+        return self
+
+    def config(self):
+        """Launch configuration of the last energies() call."""
+        out = np.zeros(4, dtype=np.int32)
+        A.check(A.lib().yh_vqe_batch_config(self._h, A.i32ptr(out)))
+        return {"tile_bits": int(out[0]), "blocks_per_cu": int(out[1]), "resident_grid": bool(out[2]),
+                "step_threads": int(out[3])}
+
     def energies(self, inst: VqeInstances, J=1.0, h=1.0):
         out = np.zeros(inst.count)
         A.check(A.lib().yh_vqe_batch_energies(self._h, inst.count, A.i32ptr(inst.nq), A.i32ptr(inst.depth),
