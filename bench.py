@@ -82,6 +82,25 @@ def cpu_baseline(els, nbits):
             "per_gate_s": per}
 
 
+def device_copy_gbps(nbytes, reps=5):
+    import torch
+    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(1)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    best = float("inf")
+    for r in range(reps + 1):
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        if r:
+            best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * nbytes / best / 1e9
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,16 +185,10 @@ def main():
     roof = roofline_of(kstats)
     if n == 28:
         roof["traffic"] = BLOCK28_TRAFFIC_BYTES
-    # practical ceiling beside the published peak: a device-to-device copy of the
-    # same state (read + write every amplitude once), fastest of 5
-    other = yaohip.ArrayReg.zero_state(n)
-    copy_s = []
-    for _ in range(6):
-        t0 = time.perf_counter()
-        A.check(L.yh_reg_copy(other._h, reg._h))  # synchronizes before returning
-        copy_s.append(time.perf_counter() - t0)
-    del other
-    copy_gbps = 2 * 16 * (1 << n) / min(copy_s[1:]) / 1e9
+    # practical ceiling beside the published peak: a device-to-device copy of a
+    # buffer of the state's size (read + write every byte once), HIP events around
+    # each copy on torch's stream, fastest of 5 after a warm-up
+    copy_gbps = device_copy_gbps(16 << n)
     roof["copy_gbps"] = round(copy_gbps, 1)
     roof["frac_of_copy"] = round(roof["achieved"] / copy_gbps, 4)
     log(f"device copy of the state: {copy_gbps:.1f} GB/s; dominant kernel at {roof['frac_of_copy']:.3f} of it")
