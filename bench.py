@@ -82,25 +82,6 @@ def cpu_baseline(els, nbits):
             "per_gate_s": per}
 
 
-def device_copy_gbps(nbytes, reps=5):
-    import torch
-    src = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
-    dst = torch.empty_like(src)
-    src.fill_(1)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    best = float("inf")
-    for r in range(reps + 1):
-        e0.record()
-        dst.copy_(src)
-        e1.record()
-        e1.synchronize()
-        if r:
-            best = min(best, e0.elapsed_time(e1) * 1e-3)
-    del src, dst
-    torch.cuda.empty_cache()
-    return 2 * nbytes / best / 1e9
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -168,12 +149,15 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(kstats[dom]["avg_us"], 2)}
 
-    def mfma_of(kstats, peak):
-        # real-split 64 x 64 product per 32 amplitudes: 64 * 64 * 2 flops per 32 amplitudes
+    def mfma_of(kstats, peak, nblocks, steps):
+        # real-split 64 x 64 product per 32 amplitudes: 64 * 64 * 2 flops per 32
+        # amplitudes and block; blocks run in k_block5 (one per launch) and k_tblocks
+        # (several per launch), so the rate is over both kernels' time
         flops = (1 << n) * 64 * 64 * 2 // 32
-        t = kstats["fused"]["avg_us"] * 1e-6
-        return {"flops_per_launch": flops, "achieved_tflops": round(flops / t / 1e12, 2), "peak_tflops": peak,
-                "frac": round(flops / t / 1e12 / peak, 4)}
+        t = sum(kstats[k]["total_ms"] for k in ("fused", "tblock") if k in kstats) * 1e-3
+        rate = flops * nblocks * steps / t
+        return {"flops_per_block": flops, "blocks_per_step": nblocks, "achieved_tflops": round(rate / 1e12, 2),
+                "peak_tflops": peak, "frac": round(rate / 1e12 / peak, 4)}
 
     gbg_dt, gbg_k = run(0, 1, 0 if args.warmup == 0 else 1, "gate-by-gate")
     tile_dt, tile_k = run(4, 1, 0 if args.warmup == 0 else 1, "tiled")
@@ -183,12 +167,14 @@ def main():
     value = ngates * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
     roof = roofline_of(kstats)
-    if n == 28:
+    if n == 28 and roof["kernel"] == "fused":
         roof["traffic"] = BLOCK28_TRAFFIC_BYTES
     # practical ceiling beside the published peak: a device-to-device copy of a
     # buffer of the state's size (read + write every byte once), HIP events around
-    # each copy on torch's stream, fastest of 5 after a warm-up
-    copy_gbps = device_copy_gbps(16 << n)
+    # each copy, fastest of 5 after a warm-up
+    gb = ctypes.c_double()
+    A.check(L.yh_copy_bandwidth(16 << n, 5, ctypes.byref(gb)))
+    copy_gbps = gb.value
     roof["copy_gbps"] = round(copy_gbps, 1)
     roof["frac_of_copy"] = round(roof["achieved"] / copy_gbps, 4)
     log(f"device copy of the state: {copy_gbps:.1f} GB/s; dominant kernel at {roof['frac_of_copy']:.3f} of it")
@@ -203,7 +189,7 @@ def main():
         "hbm_gbps_effective": round(int(bplan[0]) * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
         "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
         "roofline": roof,
-        "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS),
+        "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS, int(bplan[0]), args.steps),
         "tiled": {"value": round(ngates / tile_dt, 2), "unit": "gate-apps/s", "ms_per_step": round(1e3 * tile_dt, 3),
                   "tile_passes": int(plan[0]), "gate_groups": int(plan[1]),
                   "hbm_gbps_effective": round(int(plan[0]) * 2 * 16 * (1 << n) / tile_dt / 1e9, 1),
@@ -230,9 +216,12 @@ def main():
         for _ in range(steps64):
             yaohip.apply_inplace(circ, reg64, 5)
         dt64 = time.perf_counter() - t0
-        c, ms = ctypes.c_uint64(), ctypes.c_double()
-        A.check(L.yh_reg_timing_read(reg64._h, A.K_FUSED, ctypes.byref(c), ctypes.byref(ms)))
-        k64 = {"fused": {"launches": c.value, "avg_us": 1e3 * ms.value / max(1, c.value), "total_ms": ms.value}}
+        k64 = {}
+        for kc, kn in ((A.K_FUSED, "fused"), (A.K_TBLOCK, "tblock")):
+            c, ms = ctypes.c_uint64(), ctypes.c_double()
+            A.check(L.yh_reg_timing_read(reg64._h, kc, ctypes.byref(c), ctypes.byref(ms)))
+            if c.value:
+                k64[kn] = {"launches": c.value, "avg_us": 1e3 * ms.value / c.value, "total_ms": ms.value}
         reg64b = yaohip.ArrayReg.zero_state(n, "c64")
         yaohip.apply_inplace(circ, reg64b, 5)
         got64 = reg64b.state_vec()
@@ -241,7 +230,8 @@ def main():
         out["complex64"] = {"value": round(ngates * steps64 / dt64, 2), "unit": "gate-apps/s",
                             "ms_per_step": round(1e3 * dt64 / steps64, 3),
                             "max_rel_err_vs_fp64": err64, "fidelity_vs_fp64": fid64,
-                            "roofline": roofline_of(k64, amp_bytes=8), "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS),
+                            "roofline": roofline_of(k64, amp_bytes=8),
+                            "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS, int(bplan[0]), steps64),
                             "dtype": "f32 amplitudes and FP32 MFMA, block matrices fused in f64 on the host"}
         log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
         del reg64, reg64b, got64, ref

@@ -109,6 +109,10 @@ const char *yh_last_error(void);
 /* Selects device 0 and checks that it is a gfx950 agent. */
 int yh_init(void);
 int yh_device_name(char *buf, size_t len);
+/* Device-to-device copy rate of a `bytes`-sized buffer (read + write, GB/s), the
+ * fastest of `reps` copies timed with HIP events after one warm-up: the practical
+ * HBM ceiling the streaming kernels are compared with. No reference counterpart. */
+int yh_copy_bandwidth(size_t bytes, int reps, double *gbps);
 
 /* ---- ArrayReg (src/register.rs) ---- */
 int yh_reg_create(int nbits, int dtype, yh_reg **out);            /* zero_state :21-26  */
@@ -151,7 +155,9 @@ int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
  *   contraction (FP64 MFMA for complex128, FP32 MFMA for complex64). The planner
  *   keeps the fewest-blocks plan of several strategies and caches it by circuit
  *   structure (gate kinds and positions), so repeated calls only rebuild the
- *   block matrices.
+ *   block matrices. Blocks whose bits fit a 12-bit tile together with memory bits
+ *   0..4 share one HBM pass (up to 5 blocks applied from LDS); a block that fits
+ *   with no other streams through HBM on its own.
  * max_fuse_qubits in 1..4: multi-gate LDS tile passes (12 qubits per pass, gates
  *   applied in groups of up to 4 qubits). */
 int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
@@ -159,6 +165,11 @@ int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops
 /* Host-only plan of the 5-qubit block path: out[0] blocks, out[1] gates in blocks
  * (X split per target), out[2] gates run as their own launch. No device call. */
 int yh_block_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[3]);
+/* Host-only block plan, step by step: for step i, bits[5 i + j] is the memory bit
+ * of row bit j of the block's 32 x 32 matrix, or -1 (five times) for a gate run as
+ * its own launch. Fills at most `cap` steps; *nsteps receives the plan length.
+ * No reference counterpart (planning introspection). */
+int yh_block_plan(int nbits, const yh_op *ops, size_t nops, int32_t *bits, size_t cap, size_t *nsteps);
 /* Host-only plan of yh_apply_fused for a circuit on nbits qubits: out[0] HBM passes
  * of the tile kernel, out[1] gate groups, out[2] gates placed in tiles, out[3]
  * gates run as their own launch. No device call. */
@@ -253,7 +264,8 @@ enum yh_kclass {
   YH_K_TILE = 7,    /* multi-gate LDS tile pass          */
   YH_K_DM = 8,      /* density-matrix block pass         */
   YH_K_VQE = 9,     /* batched VQE persistent launch      */
-  YH_K_NCLASS = 10
+  YH_K_TBLOCK = 10, /* several fused blocks per LDS tile  */
+  YH_K_NCLASS = 11
 };
 int yh_reg_timing_enable(yh_reg *reg, int on);
 int yh_reg_timing_read(const yh_reg *reg, int kclass, uint64_t *launches, double *total_ms);

@@ -300,10 +300,12 @@ def test_fused_matches_gate_by_gate_26q(gpu):
 
 
 # ---------------- 5-qubit MFMA blocks (yh_apply_fused, max_fuse_qubits = 5) ----------------
-@pytest.mark.parametrize("n,seed", [(5, 0), (6, 1), (9, 2), (11, 3), (12, 4), (16, 5)])
+@pytest.mark.parametrize("n,seed", [(5, 0), (6, 1), (9, 2), (11, 3), (12, 4), (13, 6), (15, 7), (16, 5)])
 def test_block5_random_circuits_vs_oracle(gpu, n, seed):
     """All dispatch classes; gates on more than 5 qubits run on their own. n < 11
-    leaves fewer than 64 columns per block (partial workgroup steps)."""
+    leaves fewer than 64 columns per block (partial workgroup steps); from n = 13 on,
+    runs of blocks are grouped into 12-bit LDS tile passes (k_tblocks), blocks
+    moving past blocks on disjoint bits."""
     els = random_circuit(n, 200, 900 + seed, max_ctrls=3, custom_targets=(1, 2, 3, 4, 5, 6))
     psi = seeded_state(n, 50 + seed)
     print(f"[case] block5 random n={n} seed={seed}", flush=True)

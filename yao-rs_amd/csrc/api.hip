@@ -368,6 +368,36 @@ int yh_device_name(char* buf, size_t len) {
   return YH_OK;
 }
 
+int yh_copy_bandwidth(size_t bytes, int reps, double* gbps) {
+  YH_GUARD(gbps && bytes > 0 && reps >= 1, YH_ERR_INVALID_ARG, "bytes > 0, reps >= 1 and an output pointer");
+  YH_TRY(ensure_init());
+  DevBuf a, b;
+  YH_TRY(a.ensure(bytes));
+  YH_TRY(b.ensure(bytes));
+  hipStream_t st;
+  YH_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  YH_HIP(hipEventCreate(&e0));
+  YH_HIP(hipEventCreate(&e1));
+  float best = 0.0f;
+  hipError_t err = hipMemsetAsync(a.p, 1, bytes, st);
+  for (int r = 0; err == hipSuccess && r <= reps; ++r) {  // r = 0 warms up
+    err = hipEventRecord(e0, st);
+    if (err == hipSuccess) err = hipMemcpyAsync(b.p, a.p, bytes, hipMemcpyDeviceToDevice, st);
+    if (err == hipSuccess) err = hipEventRecord(e1, st);
+    if (err == hipSuccess) err = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+    if (err == hipSuccess && r > 0 && (best == 0.0f || ms < best)) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(st);
+  if (err != hipSuccess) return fail(YH_ERR_HIP, std::string("copy timing failed: ") + hipGetErrorString(err));
+  *gbps = 2.0 * double(bytes) / (double(best) * 1e-3) / 1e9;
+  return YH_OK;
+}
+
 int yh_reg_create(int nbits, int dtype, yh_reg** out) {
   YH_GUARD(out, YH_ERR_INVALID_ARG, "null output pointer");
   YH_GUARD(nbits >= 1 && nbits <= 36, YH_ERR_INVALID_ARG, "nbits must be in [1, 36]");
@@ -574,6 +604,12 @@ int yh_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]) {
   YH_GUARD(out, YH_ERR_INVALID_ARG, "null argument");
   YH_TRY(yh_validate(nbits, ops, nops));
   return plan_stats(nbits, ops, nops, out);
+}
+
+int yh_block_plan(int nbits, const yh_op* ops, size_t nops, int32_t* bits, size_t cap, size_t* nsteps) {
+  YH_GUARD(nsteps, YH_ERR_INVALID_ARG, "null argument");
+  YH_TRY(yh_validate(nbits, ops, nops));
+  return block_plan_bits(nbits, ops, nops, bits, cap, nsteps);
 }
 
 int yh_block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]) {

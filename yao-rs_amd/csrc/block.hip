@@ -1,6 +1,7 @@
 // block.hip — yh_apply_fused with max_fuse_qubits = 5: gates fused into 5-qubit
-// blocks (32 x 32 complex matrices, multiplied in FP64 on the host), each applied
-// in one HBM pass as a real-split MFMA contraction.
+// blocks (32 x 32 complex matrices, multiplied in FP64 on the host), applied as
+// real-split MFMA contractions: a block on its own in one HBM pass (k_block5), or
+// several blocks whose bits fit one 12-bit LDS tile in one pass (k_tblocks).
 //
 // Planner: a block starts at the first gate not yet applied and takes, in circuit
 // order, every gate whose qubits (targets and controls) keep the block within 5
@@ -177,6 +178,154 @@ __global__ __launch_bounds__(256, 3) void k_block5(typename Cplx<R>::V* __restri
     o.x = prev[m >> 2][m & 3];
     o.y = prev[2 + (m >> 2)][m & 3];
     s[bprev | out_lane | out_u[m]] = o;
+  }
+}
+
+// ---- several blocks per HBM pass: an LDS tile of 2^12 amplitudes ----
+// The tile holds memory bits 0..4 (runs of 32 consecutive amplitudes) plus 7 more;
+// every block of the pass has its 5 bits among them. One 512-thread workgroup per
+// CU keeps the tile and the A values of all the pass's blocks in LDS (the A values
+// once per launch). Per tile: the amplitudes, loaded into registers during the
+// previous tile, go to LDS; the next tile's loads are issued; each block is the
+// same real-split MFMA product as k_block5 on the tile viewed as a 32 x 128 matrix
+// (one 16-column step per wave), with a barrier between blocks; the tile is stored.
+// Barriers inside the loop order LDS only, so the next tile's loads stay in flight
+// while the MFMAs run. U's real-split matrix [[Ur, -Ui], [Ui, Ur]] only holds Ur
+// and Ui, so 32 A values per lane are kept, not 64: the -Ui products are taken as
+// Ui times the negated imaginary parts, which rounds exactly as k_block5 does.
+constexpr int kTT = 12;                // tile bits
+constexpr int kTN = 1 << kTT;          // 4096 amplitudes
+constexpr int kTHigh = kTT - kBQ;      // 7 tile bits above the low 5
+constexpr int kTThreads = 512;         // 8 waves: one 16-column step each per block
+constexpr int kPassMax = 5;            // blocks per pass
+constexpr int kCFrag = 32 * 64;        // compact A values of one block
+
+struct TBlockArgs {
+  uint8_t tbit[kTT];          // memory bit of tile position k, ascending (0..4 first)
+  uint8_t q[kPassMax][kBQ];   // tile position of row bit j of block b
+  uint8_t qs[kPassMax][kBQ];  // the same positions, ascending
+  int32_t nblk;
+  uint64_t ntiles;            // 2^(n - 12)
+};
+
+__device__ __forceinline__ int tpad(int p) { return p + (p >> 4); }
+
+template <class R>
+__global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* __restrict__ s, TBlockArgs a,
+                                                        const R* __restrict__ frag) {
+  using V = typename Cplx<R>::V;
+  using M = MfmaOps<R>;
+  __shared__ V lds[kTN + kTN / 16];
+  __shared__ R afr[kPassMax * kCFrag];
+  __shared__ uint64_t hoff[1 << kTHigh];
+  __shared__ int bq[kPassMax][2 * kBQ];  // per block: q[0..4], then qs[0..4]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  if (tid < (1 << kTHigh)) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int k = 0; k < kTHigh; ++k)
+      if ((tid >> k) & 1) o |= uint64_t(1) << a.tbit[kBQ + k];
+    hoff[tid] = o;
+  }
+  if (tid < kPassMax * kBQ) {
+    const int b = tid / kBQ, j = tid % kBQ;
+    bq[b][j] = a.q[b][j];
+    bq[b][kBQ + j] = a.qs[b][j];
+  }
+  for (int e = tid; e < a.nblk * kCFrag; e += kTThreads) afr[e] = frag[e];
+  __syncthreads();
+  constexpr int kPer = kTN / kTThreads;
+  uint64_t t = blockIdx.x;
+  if (t >= a.ntiles) return;  // uniform
+  uint64_t base = insert_zeros(t, a.tbit, kTT);
+  R vx[kPer], vy[kPer];  // scalar arrays: kept in registers across the loop
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int p = tid + kTThreads * k;
+    const V x = s[base + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]];
+    vx[k] = x.x;
+    vy[k] = x.y;
+  }
+  for (;;) {
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
+    lds_barrier();
+    const uint64_t tn = t + gridDim.x;
+    const bool more = tn < a.ntiles;  // uniform
+    // the last tile re-reads itself (unused): unconditional loads keep v in registers
+    const uint64_t bn = more ? insert_zeros(tn, a.tbit, kTT) : base;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + kTThreads * k;
+      const V x = s[bn + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]];
+      vx[k] = x.x;
+      vy[k] = x.y;
+    }
+    for (int b = 0; b < a.nblk; ++b) {
+      const R* f = afr + b * kCFrag;
+      R ur[16], ui[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        ur[e] = f[e * 64 + lane];
+        ui[e] = f[(16 + e) * 64 + lane];
+      }
+      int q[kBQ], qs[kBQ];
+#pragma unroll
+      for (int j = 0; j < kBQ; ++j) {
+        q[j] = bq[b][j];
+        qs[j] = bq[b][kBQ + j];
+      }
+      auto roff = [&](int r) {
+        int o = 0;
+#pragma unroll
+        for (int j = 0; j < kBQ; ++j) o |= ((r >> j) & 1) << q[j];
+        return o;
+      };
+      const int in_lane = roff(grp), out_lane = roff(M::drow(lane, 0));
+      int cb = wave * kCols + col;  // this lane's column, then its tile position
+#pragma unroll
+      for (int j = 0; j < kBQ; ++j) cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
+      V x[8];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(cb | in_lane | roff(4 * m))];
+      typename M::acc_t acc[4];
+#pragma unroll
+      for (int rt = 0; rt < 4; ++rt) acc[rt] = typename M::acc_t{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const R br = x[ks].x;
+        acc[0] = M::mfma(ur[ks], br, acc[0]);
+        acc[1] = M::mfma(ur[8 + ks], br, acc[1]);
+        acc[2] = M::mfma(ui[ks], br, acc[2]);
+        acc[3] = M::mfma(ui[8 + ks], br, acc[3]);
+      }
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const R bi = x[ks].y, nbi = -x[ks].y;
+        acc[0] = M::mfma(ui[ks], nbi, acc[0]);
+        acc[1] = M::mfma(ui[8 + ks], nbi, acc[1]);
+        acc[2] = M::mfma(ur[ks], bi, acc[2]);
+        acc[3] = M::mfma(ur[8 + ks], bi, acc[3]);
+      }
+#pragma unroll
+      for (int m = 0; m < 8; ++m) {
+        V o;
+        o.x = acc[m >> 2][m & 3];
+        o.y = acc[2 + (m >> 2)][m & 3];
+        lds[tpad(cb | out_lane | roff(16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0))))] = o;
+      }
+      lds_barrier();
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = tid + kTThreads * k;
+      s[base + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]] = lds[tpad(p)];
+    }
+    lds_barrier();  // the next tile refills LDS
+    if (!more) break;
+    t = tn;
+    base = bn;
   }
 }
 
@@ -501,6 +650,79 @@ void to_fragments(const std::vector<cd>& U, R* o) {
   (void)RD;
 }
 
+// compact A values for k_tblocks: value e of lane l is Ur (e < 16) or Ui (e >= 16)
+// at row 16 (e % 16 / 8) + l % 16, column 4 (e % 8) + l / 16
+template <class R>
+void to_compact(const std::vector<cd>& U, R* o) {
+  for (int e = 0; e < 32; ++e)
+    for (int l = 0; l < 64; ++l) {
+      const int r = 16 * ((e % 16) / 8) + l % 16, c = 4 * (e % 8) + l / 16;
+      const cd u = U[size_t(r) * kBD + c];
+      o[e * 64 + l] = R(e < 16 ? u.re : u.im);
+    }
+}
+
+// Launch order of a block plan: gates run alone stay where they are; each run of
+// consecutive blocks is split into passes of k_tblocks. A pass starts at the first
+// block not yet placed and takes, in order, every later block whose bits keep the
+// tile (bits 0..4 plus the pass's block bits) within 12 bits and that shares no bit
+// with a block skipped before it, so blocks only move past blocks on disjoint bits.
+struct PassItem {
+  bool single = false;
+  size_t step = 0;              // single: index into the plan
+  std::vector<size_t> blocks;   // pass: plan indices, in application order
+  uint64_t bits = 0;            // pass: tile bits (before padding)
+};
+
+uint64_t step_mask(const BStep& st) {
+  uint64_t m = 0;
+  for (int j = 0; j < kBQ; ++j) m |= uint64_t(1) << st.p[j];
+  return m;
+}
+
+std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
+  std::vector<PassItem> items;
+  size_t i = 0;
+  while (i < steps.size()) {
+    if (!steps[i].block) {
+      PassItem it;
+      it.single = true;
+      it.step = i++;
+      items.push_back(it);
+      continue;
+    }
+    size_t j = i;
+    while (j < steps.size() && steps[j].block) ++j;
+    std::vector<size_t> rem;
+    for (size_t k = i; k < j; ++k) rem.push_back(k);
+    while (!rem.empty()) {
+      PassItem it;
+      uint64_t cur = (uint64_t(1) << kBQ) - 1, blocked = 0;
+      std::vector<size_t> left;
+      for (size_t k : rem) {
+        const uint64_t bm = step_mask(steps[k]);
+        if (nbits > kTT && !(bm & blocked) && popc(cur | bm) <= kTT && it.blocks.size() < size_t(kPassMax)) {
+          cur |= bm;
+          it.blocks.push_back(k);
+        } else {
+          blocked |= bm;
+          left.push_back(k);
+        }
+      }
+      if (it.blocks.empty()) {  // n <= 12: one block per pass (k_block5)
+        it.blocks.push_back(left.front());
+        cur |= step_mask(steps[left.front()]);
+        left.erase(left.begin());
+      }
+      it.bits = cur;
+      items.push_back(std::move(it));
+      rem.swap(left);
+    }
+    i = j;
+  }
+  return items;
+}
+
 }  // namespace
 
 template <class R>
@@ -510,37 +732,51 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
   std::vector<BStep> steps;
   std::vector<BGate> bg;
   YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
-  size_t nblk = 0;
-  for (auto& s : steps) nblk += s.block;
-  // persistent grid: as many workgroups as the CUs hold at once
-  static int resident[2] = {0, 0};
-  int& res = resident[sizeof(R) == 8 ? 0 : 1];
-  if (res == 0) {
-    int dev = 0, per_cu = 0;
+  // persistent grids: as many workgroups as the CUs hold at once
+  static int resident[2][2] = {{0, 0}, {0, 0}};
+  int(&res)[2] = resident[sizeof(R) == 8 ? 0 : 1];
+  if (res[0] == 0) {
+    int dev = 0, per_cu = 0, per_cu_t = 0;
     hipDeviceProp_t prop;
     YH_HIP(hipGetDevice(&dev));
     YH_HIP(hipGetDeviceProperties(&prop, dev));
     YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block5<R>, 256, 0));
-    res = prop.multiProcessorCount * std::max(1, per_cu);
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_tblocks<R>, kTThreads, 0));
+    res[0] = prop.multiProcessorCount * std::max(1, per_cu);
+    res[1] = prop.multiProcessorCount * std::max(1, per_cu_t);
   }
-  // Block matrices are built on the host while the GPU runs the previous blocks:
-  // each goes to its own slot of a pinned staging area and is copied with an
-  // async copy ordered before its launch (nothing else writes the area in this call).
-  const size_t slot = kFrag * sizeof(R);
+  const std::vector<PassItem> items = group_passes(steps, nbits);
+  // Block matrices are built on the host while the GPU runs the previous passes:
+  // each pass's operands go to their own part of a pinned staging area and are
+  // copied with an async copy ordered before its launch (nothing else writes the
+  // area in this call).
+  size_t total = 0;
+  for (const PassItem& it : items)
+    if (!it.single) total += it.blocks.size() == 1 ? kFrag : it.blocks.size() * kCFrag;
   YH_TRY(w.sync());
-  YH_TRY(w.bpin.ensure(std::max<size_t>(1, nblk) * slot));
-  YH_TRY(w.tiles.ensure(std::max<size_t>(1, nblk) * slot));
-  unsigned char* hst = static_cast<unsigned char*>(w.bpin.p);
-  unsigned char* dev = static_cast<unsigned char*>(w.tiles.p);
+  YH_TRY(w.bpin.ensure(std::max<size_t>(1, total) * sizeof(R)));
+  YH_TRY(w.tiles.ensure(std::max<size_t>(1, total) * sizeof(R)));
+  R* hst = static_cast<R*>(w.bpin.p);
+  R* dev = static_cast<R*>(w.tiles.p);
   V* s = static_cast<V*>(state);
   std::vector<cd> U;
-  size_t bi = 0;
-  for (const BStep& st : steps) {
-    if (st.block) {
+  size_t off = 0;
+  for (const PassItem& it : items) {
+    if (it.single) {
+      const BGate& g = bg[steps[it.step].gate];
+      yh_op one = ops[g.op_index];
+      if (one.kind == YH_GATE_X && one.nt > 1) {
+        one.targets[0] = one.targets[g.sub];
+        one.nt = 1;
+      }
+      YH_TRY(apply_gates<R>(w, state, nbits, &one, 1));
+      continue;
+    }
+    if (it.blocks.size() == 1) {
+      const BStep& st = steps[it.blocks[0]];
       block_matrix(st, bg, U);
-      R* frag = reinterpret_cast<R*>(hst + bi * slot);
-      to_fragments<R>(U, frag);
-      YH_HIP(hipMemcpyAsync(dev + bi * slot, frag, slot, hipMemcpyHostToDevice, w.stream));
+      to_fragments<R>(U, hst + off);
+      YH_HIP(hipMemcpyAsync(dev + off, hst + off, kFrag * sizeof(R), hipMemcpyHostToDevice, w.stream));
       BlockArgs a{};
       int sp[kBQ];
       std::memcpy(sp, st.p, sizeof(sp));
@@ -551,28 +787,67 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
       }
       a.ncol = uint64_t(1) << (nbits - kBQ);
       a.nchunk = (a.ncol + kCols - 1) / kCols;
-      const uint64_t grid = std::min<uint64_t>((a.nchunk + 3) / 4, uint64_t(res));
+      const uint64_t grid = std::min<uint64_t>((a.nchunk + 3) / 4, uint64_t(res[0]));
       YH_TRY(w.time_begin(YH_K_FUSED));
-      hipLaunchKernelGGL(k_block5<R>, dim3(unsigned(grid)), dim3(256), 0, w.stream, s, a,
-                         reinterpret_cast<const R*>(dev + bi * slot));
+      hipLaunchKernelGGL(k_block5<R>, dim3(unsigned(grid)), dim3(256), 0, w.stream, s, a, dev + off);
       YH_HIP(hipGetLastError());
       YH_TRY(w.time_end());
-      ++bi;
-    } else {
-      const BGate& g = bg[st.gate];
-      yh_op one = ops[g.op_index];
-      if (one.kind == YH_GATE_X && one.nt > 1) {
-        one.targets[0] = one.targets[g.sub];
-        one.nt = 1;
-      }
-      YH_TRY(apply_gates<R>(w, state, nbits, &one, 1));
+      off += kFrag;
+      continue;
     }
+    // a multi-block pass: tile bits = the pass's bits padded with the lowest free
+    // bits to exactly 12
+    TBlockArgs a{};
+    uint64_t tb = it.bits;
+    for (int b = 0; popc(tb) < kTT && b < nbits; ++b) tb |= uint64_t(1) << b;
+    if (popc(tb) != kTT || nbits <= kTT) return fail(YH_ERR_UNSUPPORTED, "block pass tile does not fit");
+    int pos_of[64];
+    for (int b = 0, k = 0; b < nbits; ++b)
+      if ((tb >> b) & 1) {
+        a.tbit[k] = uint8_t(b);
+        pos_of[b] = k++;
+      }
+    a.nblk = int(it.blocks.size());
+    a.ntiles = uint64_t(1) << (nbits - kTT);
+    for (size_t bi = 0; bi < it.blocks.size(); ++bi) {
+      const BStep& st = steps[it.blocks[bi]];
+      int qs[kBQ];
+      for (int j = 0; j < kBQ; ++j) {
+        a.q[bi][j] = uint8_t(pos_of[st.p[j]]);
+        qs[j] = pos_of[st.p[j]];
+      }
+      std::sort(qs, qs + kBQ);
+      for (int j = 0; j < kBQ; ++j) a.qs[bi][j] = uint8_t(qs[j]);
+      block_matrix(st, bg, U);
+      to_compact<R>(U, hst + off + bi * kCFrag);
+    }
+    const size_t sz = it.blocks.size() * kCFrag;
+    YH_HIP(hipMemcpyAsync(dev + off, hst + off, sz * sizeof(R), hipMemcpyHostToDevice, w.stream));
+    const uint64_t grid = std::min<uint64_t>(a.ntiles, uint64_t(res[1]));
+    YH_TRY(w.time_begin(YH_K_TBLOCK));
+    hipLaunchKernelGGL(k_tblocks<R>, dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a, dev + off);
+    YH_HIP(hipGetLastError());
+    YH_TRY(w.time_end());
+    off += sz;
   }
   return w.sync();
 }
 
 template int apply_blocks<double>(Workspace&, void*, int, const yh_op*, size_t);
 template int apply_blocks<float>(Workspace&, void*, int, const yh_op*, size_t);
+
+// every step of the block plan: 5 memory bits (row bit j <-> p[j]) per block, or
+// five -1 for a gate run as its own launch
+int block_plan_bits(int nbits, const yh_op* ops, size_t nops, int32_t* bits, size_t cap, size_t* nsteps) {
+  std::vector<BStep> steps;
+  std::vector<BGate> bg;
+  YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
+  *nsteps = steps.size();
+  if (bits)
+    for (size_t i = 0; i < steps.size() && i < cap; ++i)
+      for (int j = 0; j < kBQ; ++j) bits[i * kBQ + j] = steps[i].block ? steps[i].p[j] : -1;
+  return YH_OK;
+}
 
 // out[0] blocks, out[1] gates (X split per target) in blocks, out[2] single launches
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]) {

@@ -76,6 +76,7 @@ int plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[4]);
 template <class R>
 int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]);
+int block_plan_bits(int nbits, const yh_op* ops, size_t nops, int32_t* bits, size_t cap, size_t* nsteps);
 
 }  // namespace yh
 

@@ -30,8 +30,8 @@ GATE_NAMES = ["X", "Y", "Z", "H", "S", "T", "SWAP", "Phase", "Rx", "Ry", "Rz", "
  CH_CUSTOM) = range(11)
 OP_I, OP_X, OP_Y, OP_Z, OP_P0, OP_P1, OP_PU, OP_PD = range(8)
 MLP_LINEAR, MLP_TRIG = 0, 1
-(K_1Q, K_DENSE, K_DIAG, K_X, K_SWAP, K_OOP, K_FUSED, K_TILE, K_DM, K_VQE) = range(10)
-KCLASS_NAMES = ["1q", "dense", "diag", "x", "swap", "oop", "fused", "tile", "dm", "vqe"]
+(K_1Q, K_DENSE, K_DIAG, K_X, K_SWAP, K_OOP, K_FUSED, K_TILE, K_DM, K_VQE, K_TBLOCK) = range(11)
+KCLASS_NAMES = ["1q", "dense", "diag", "x", "swap", "oop", "fused", "tile", "dm", "vqe", "tblock"]
 MAX_TARGETS = 12
 MAX_CTRLS = 16
 
@@ -113,6 +113,7 @@ SIGNATURES = [
     ("yh_last_error", ctypes.c_char_p, []),
     ("yh_init", c_int, []),
     ("yh_device_name", c_int, [ctypes.c_char_p, c_size_t]),
+    ("yh_copy_bandwidth", c_int, [c_size_t, c_int, P(ctypes.c_double)]),
     ("yh_reg_create", c_int, [c_int, c_int, P(c_void_p)]),
     ("yh_reg_destroy", c_int, [c_void_p]),
     ("yh_reg_nbits", c_int, [c_void_p, P(c_int)]),
@@ -137,6 +138,7 @@ SIGNATURES = [
                                P(ctypes.c_int32), P(ctypes.c_int32), P(c_double), P(c_double), c_size_t]),
     ("yh_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
     ("yh_block_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
+    ("yh_block_plan", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int32), c_size_t, P(c_size_t)]),
     ("yh_gate_matrix", c_int, [P(YhOp), P(c_double), P(c_int)]),
     ("yh_expect", c_int, [c_void_p, c_size_t, P(c_double), P(ctypes.c_int32),
                           P(ctypes.c_int32), P(ctypes.c_int32), P(c_double)]),
