@@ -29,12 +29,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBPS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
-# HBM bytes per launch of k_block5<double> on the 28-qubit state, from rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over the 48 launches
-# of one step), each scaled by the factor a plain copy of known size gives for the
-# same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
+# HBM bytes per launch on the 28-qubit state, from rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 11 of k_tblocks,
+# 20 of k_block5), each scaled by the factor a plain copy of known size gives for
+# the same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
 # profiles/k_block5_brickwork28_pmc.md.
-BLOCK28_TRAFFIC_BYTES = int((2 * 2097825 + 4256224) * 1024)
+TRAFFIC28_BYTES = {"tblock": int((2 * 2097575 + 4194304) * 1024),
+                   "fused": int((2 * 2097501 + 4194304) * 1024)}
 NBITS = 28
 DEPTH = 20
 SEED = 20
@@ -140,9 +141,9 @@ def main():
         log(f"{label}: {dt / steps * 1e3:.2f} ms/step; norm {reg.norm():.15f}; kernels {kstats}")
         return dt, kstats
 
-    def roofline_of(kstats, amp_bytes=16):
+    def roofline_of(kstats, amp_bytes=16, dom=None):
         # dominant kernel; every pass type reads and writes every amplitude once
-        dom = max(kstats, key=lambda k: kstats[k]["total_ms"])
+        dom = dom or max(kstats, key=lambda k: kstats[k]["total_ms"])
         bytes_per_launch = 2 * amp_bytes * (1 << n)
         achieved = bytes_per_launch / (kstats[dom]["avg_us"] * 1e-6) / 1e9
         return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -166,18 +167,36 @@ def main():
     A.check(L.yh_block_plan_stats(n, ol.arr, ol.n, A.i64ptr(bplan)))
     value = ngates * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
-    roof = roofline_of(kstats)
-    if n == 28 and roof["kernel"] == "fused":
-        roof["traffic"] = BLOCK28_TRAFFIC_BYTES
-    # practical ceiling beside the published peak: a device-to-device copy of a
-    # buffer of the state's size (read + write every byte once), HIP events around
-    # each copy, fastest of 5 after a warm-up
+    # The two block kernels: k_block5 (a block alone in its HBM pass, HBM-bound)
+    # and k_tblocks (several blocks per pass applied from LDS, MFMA-bound); the
+    # roofline reported is the one of the kernel with the larger share of the step.
     gb = ctypes.c_double()
     A.check(L.yh_copy_bandwidth(16 << n, 5, ctypes.byref(gb)))
-    copy_gbps = gb.value
+    copy_gbps = gb.value  # device-to-device copy of a state-sized buffer, HIP events
+    log(f"device copy of the state: {copy_gbps:.1f} GB/s")
+    flops_block = (1 << n) * 64 * 64 * 2 // 32
+    per_kernel = {}
+    if "fused" in kstats:
+        r = roofline_of(kstats, dom="fused")
+        r["traffic"] = TRAFFIC28_BYTES["fused"] if n == 28 else None
+        r["copy_gbps"] = round(copy_gbps, 1)
+        r["frac_of_copy"] = round(r["achieved"] / copy_gbps, 4)
+        per_kernel["k_block5"] = r
+    if "tblock" in kstats:
+        t = kstats["tblock"]
+        nb_tile = int(bplan[0]) * args.steps - kstats.get("fused", {}).get("launches", 0)
+        bpl = nb_tile / t["launches"]
+        tf = bpl * flops_block / (t["avg_us"] * 1e-6) / 1e12
+        per_kernel["k_tblocks"] = {
+            "bound": "mfma", "achieved": round(tf, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
+            "traffic": TRAFFIC28_BYTES["tblock"] if n == 28 else None, "kernel": "tblock",
+            "blocks_per_launch": round(bpl, 3), "flops_per_launch": int(bpl * flops_block),
+            "avg_launch_us": round(t["avg_us"], 2),
+            "hbm_gbps": round(2 * 16 * (1 << n) / (t["avg_us"] * 1e-6) / 1e9, 1)}
+    dom = max((k for k in ("fused", "tblock") if k in kstats), key=lambda k: kstats[k]["total_ms"])
+    roof = dict(per_kernel["k_block5" if dom == "fused" else "k_tblocks"])
     roof["copy_gbps"] = round(copy_gbps, 1)
-    roof["frac_of_copy"] = round(roof["achieved"] / copy_gbps, 4)
-    log(f"device copy of the state: {copy_gbps:.1f} GB/s; dominant kernel at {roof['frac_of_copy']:.3f} of it")
     out = {
         "metric": "gate-apps/s at 28q c128 (random brickwork depth 20)",
         "value": round(value, 2), "unit": "gate-apps/s", "n_gpus": 1, "steps": args.steps,
@@ -189,6 +208,7 @@ def main():
         "hbm_gbps_effective": round(int(bplan[0]) * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
         "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
         "roofline": roof,
+        "roofline_kernels": per_kernel,
         "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS, int(bplan[0]), args.steps),
         "tiled": {"value": round(ngates / tile_dt, 2), "unit": "gate-apps/s", "ms_per_step": round(1e3 * tile_dt, 3),
                   "tile_passes": int(plan[0]), "gate_groups": int(plan[1]),
