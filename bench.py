@@ -385,31 +385,57 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
 def bench_dm(log, n=14, depth=10):
     """14-qubit noisy density matrix (2^28 complex128 entries), brickwork depth 10 with
     Depolarizing(1e-3) then AmplitudeDamping(1e-3) after every gate on each touched
-    qubit; gate and its channels fused into one pass. Outputs <Z_i Z_i+1>, trace, purity."""
+    qubit. Reported path: gates and channels as superoperators on the 28-bit vector
+    rho, fused into 5-bit MFMA blocks (yh_dm_apply_blocks); beside it the per-gate
+    path that fuses each gate with its channels into one pass (yh_dm_apply_fused).
+    Outputs <Z_i Z_i+1>, trace, purity, and the largest difference between the paths."""
     import yaohip
     from yaohip import _abi as A
-    from yaohip.circuit import Circuit, OperatorPolynomial, OperatorString, Op
+    from yaohip.circuit import Circuit
     from fixture_circuits import noisy_brickwork
     els = noisy_brickwork(n, depth, 3)
     circ = Circuit.qubits(n, els)
     ngates = sum(1 for e in els if hasattr(e, "gate"))
     nch = len(els) - ngates
-    dm = yaohip.DensityMatrix.zero_state(n)
-    A.check(A.lib().yh_dm_timing_enable(dm._h, 1))
-    t0 = time.perf_counter()
-    dm.apply(circ, fused=True)
-    dt = time.perf_counter() - t0
-    c, ms = ctypes.c_uint64(), ctypes.c_double()
-    A.check(A.lib().yh_dm_timing_read(dm._h, A.K_DM, ctypes.byref(c), ctypes.byref(ms)))
-    tr = dm.trace()
-    pur = dm.purity()
-    zz = [gpu_zz(dm, i) for i in range(n - 1)]
     bytes_pass = 2 * 16 * (1 << (2 * n))
-    avg = ms.value / max(c.value, 1)
-    res = {"nbits": n, "depth": depth, "gates": ngates, "channels": nch, "passes": int(c.value),
-           "elements_per_s": round(len(els) / dt, 1), "seconds": round(dt, 4),
-           "avg_pass_us": round(1e3 * avg, 2), "pass_gbps": round(bytes_pass / (avg * 1e-3) / 1e9, 1),
-           "trace": [tr.real, tr.imag], "purity": pur, "zz_lsb_sites": zz}
+
+    def run(mode):
+        dm = yaohip.DensityMatrix.zero_state(n)
+        dm.apply(circ, fused=mode)  # warm-up (plan, first launches)
+        dm = yaohip.DensityMatrix.zero_state(n)
+        A.check(A.lib().yh_dm_timing_enable(dm._h, 1))
+        t0 = time.perf_counter()
+        dm.apply(circ, fused=mode)
+        dt = time.perf_counter() - t0
+        kern = {}
+        for kc, kn in ((A.K_DM, "dm"), (A.K_FUSED, "fused"), (A.K_TBLOCK, "tblock")):
+            c, ms = ctypes.c_uint64(), ctypes.c_double()
+            A.check(A.lib().yh_dm_timing_read(dm._h, kc, ctypes.byref(c), ctypes.byref(ms)))
+            if c.value:
+                kern[kn] = {"launches": int(c.value), "avg_us": round(1e3 * ms.value / c.value, 2),
+                            "total_ms": round(ms.value, 3)}
+        return dm, dt, kern
+
+    dm, dt, kern = run("blocks")
+    tr, pur = dm.trace(), dm.purity()
+    zz = [gpu_zz(dm, i) for i in range(n - 1)]
+    del dm
+    dmf, dtf, kernf = run(True)
+    trf, purf = dmf.trace(), dmf.purity()
+    zzf = [gpu_zz(dmf, i) for i in range(n - 1)]
+    del dmf
+    passes = sum(k["launches"] for k in kern.values())
+    res = {"nbits": n, "depth": depth, "gates": ngates, "channels": nch,
+           "path": "superoperator blocks (yh_dm_apply_blocks)", "passes": passes,
+           "elements_per_s": round(len(els) / dt, 1), "seconds": round(dt, 4), "kernels": kern,
+           "pass_bytes": bytes_pass, "hbm_gbps_effective": round(passes * bytes_pass / dt / 1e9, 1),
+           "trace": [tr.real, tr.imag], "purity": pur, "zz_lsb_sites": zz,
+           "per_gate_fused": {"passes": sum(k["launches"] for k in kernf.values()), "seconds": round(dtf, 4),
+                              "elements_per_s": round(len(els) / dtf, 1), "kernels": kernf,
+                              "pass_gbps": round(bytes_pass / (kernf["dm"]["avg_us"] * 1e-6) / 1e9, 1)},
+           "speedup_vs_per_gate_fused": round(dtf / dt, 2),
+           "max_abs_diff_vs_per_gate": max([abs(tr - trf), abs(pur - purf)] +
+                                           [abs(a - b) for a, b in zip(zz, zzf)])}
     log(f"noisy dm: {res}")
     return res
 

@@ -227,6 +227,14 @@ int yh_dm_apply(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
 /* Same result; each gate is fused with the 1-qubit channels that follow it on its
  * target qubits into one superoperator pass. */
 int yh_dm_apply_fused(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
+/* Same result through the 5-qubit block path: rho is a vector of 2n bits (row bits
+ * above column bits), a gate is U (x) conj(U) on the row and column bits of its
+ * qubits and a channel sum_k K_k (x) conj(K_k); these superoperators (at most 4
+ * bits for 1- and 2-qubit elements) are fused into 5-bit blocks by the planner of
+ * yh_apply_fused and applied with the same MFMA kernels, several blocks per HBM
+ * pass where they share a 12-bit tile. Elements on more than 2 qubits run on their
+ * own as in yh_dm_apply. */
+int yh_dm_apply_blocks(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
 int yh_dm_trace(const yh_dm *dm, double out[2]);                   /* trace :63-67      */
 int yh_dm_purity(const yh_dm *dm, double *out);                    /* purity :69-78     */
 /* partial_tr (density_matrix.rs:80-111): the reduced density matrix over the

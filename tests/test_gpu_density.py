@@ -20,7 +20,7 @@ def _mat(d):
     return np.array(d["re"]) + 1j * np.array(d["im"])
 
 
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [False, True, "blocks"])
 def test_dm_cases_match_julia(gpu, fused):
     fx = load_golden("noise.json")["dm_cases"]
     cases = {
@@ -66,7 +66,7 @@ def random_noisy(n, ngates, seed):
     return out
 
 
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [False, True, "blocks"])
 @pytest.mark.parametrize("n,seed", [(1, 0), (3, 1), (5, 2), (7, 3)])
 def test_random_noisy_circuits_vs_oracle(gpu, n, seed, fused):
     els = random_noisy(n, 40, 100 + seed)
@@ -91,7 +91,7 @@ def test_random_noisy_circuits_vs_oracle(gpu, n, seed, fused):
     assert np.array_equal(gpu.probs(dm, locs), O.dm_probs(n, got, locs))
 
 
-@pytest.mark.parametrize("fused", [False, True])
+@pytest.mark.parametrize("fused", [False, True, "blocks"])
 def test_noisy_brickwork_8q(gpu, fused):
     """The noisy brickwork workload at 8 qubits, depth 3."""
     n = 8
@@ -130,6 +130,23 @@ def test_dm_large_trace_and_purity_14q(gpu):
     assert abs(dm.trace() - 1) < 1e-12
     p = dm.purity()
     assert 0.9 < p < 1.0
+
+
+def test_dm_blocks_match_fused_14q(gpu):
+    """The superoperator block path (12-bit tile passes at 2^28 entries) against the
+    per-gate fused path on the noisy brickwork: trace, purity and every ZZ."""
+    n = 14
+    els = noisy_brickwork(n, 2, 3)
+    print("[case] dm blocks vs fused 14q", flush=True)
+    a = gpu.DensityMatrix.zero_state(n)
+    a.apply(Circuit.qubits(n, els), fused=True)
+    b = gpu.DensityMatrix.zero_state(n)
+    b.apply(Circuit.qubits(n, els), fused="blocks")
+    assert abs(b.trace() - 1) < 1e-12
+    assert abs(b.purity() - a.purity()) < 1e-12
+    for i in range(n - 1):
+        zz = OperatorPolynomial([1.0], [OperatorString([(i, Op.Z), (i + 1, Op.Z)])])
+        assert abs(gpu.expect_dm(b, zz) - gpu.expect_dm(a, zz)) < 1e-12
 
 
 def reference_noisy_circuit(nq):

@@ -1452,6 +1452,26 @@ int yh_dm_apply_fused(yh_dm* dm, int circuit_nbits, const yh_op* ops, size_t nop
   return dm_run(dm, L, arena);
 }
 
+int yh_dm_apply_blocks(yh_dm* dm, int circuit_nbits, const yh_op* ops, size_t nops) {
+  YH_GUARD(dm, YH_ERR_INVALID_ARG, "null density matrix");
+  YH_GUARD(circuit_nbits == dm->nbits, YH_ERR_NBITS_MISMATCH, "Register and circuit qubit count mismatch");
+  YH_TRY(yh_validate(circuit_nbits, ops, nops));
+  for (size_t e = 0; e < nops; ++e)  // the same parameter checks as the other paths, before any launch
+    if (ops[e].elem == YH_ELEM_CHANNEL) {
+      std::vector<Mat> kr;
+      int d = 0;
+      if (kraus_operators(ops[e], kr, d) != YH_OK) return fail(YH_ERR_CHANNEL_PARAM, "invalid channel parameters");
+      if (ops[e].nt > 2) return fail(YH_ERR_UNSUPPORTED, "density-matrix channels act on at most 2 qubits here");
+    }
+  return apply_dm_blocks(dm->w, dm->rho.p, dm->nbits, ops, nops, [&](size_t e) {
+    std::vector<DmLaunch> L;
+    Arena<double> arena;
+    if (ops[e].elem == YH_ELEM_GATE) YH_TRY(dm_plan_gate(dm->nbits, ops[e], L, arena));
+    else YH_TRY(dm_plan_channel(dm->nbits, ops[e], L, arena));
+    return L.empty() ? YH_OK : dm_run(dm, L, arena);
+  });
+}
+
 int yh_dm_trace(const yh_dm* cdm, double out[2]) {
   YH_GUARD(cdm && out, YH_ERR_INVALID_ARG, "null argument");
   yh_dm* dm = const_cast<yh_dm*>(cdm);

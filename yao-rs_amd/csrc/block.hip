@@ -21,6 +21,7 @@
 // barrier in the loop.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -579,9 +580,9 @@ PlanCache& plan_cache() {
   return c;
 }
 
-int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& steps, std::vector<BGate>& bg) {
-  YH_TRY(to_bgates(nbits, ops, nops, bg));
-  const uint64_t key = structure_key(nbits, ops, nops);
+// fewest-blocks plan of a gate list, cached by `key` (the structure of the ops the
+// gate list was built from)
+int plan_gates(const std::vector<BGate>& bg, int nbits, uint64_t key, size_t nops, std::vector<BStep>& steps) {
   {
     PlanCache& c = plan_cache();
     std::lock_guard<std::mutex> lk(c.mu);
@@ -609,6 +610,83 @@ int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& st
   std::lock_guard<std::mutex> lk(c.mu);
   c.entries.push_back({key, nbits, nops, std::move(bestp)});
   if (c.entries.size() > 8) c.entries.erase(c.entries.begin());
+  return YH_OK;
+}
+
+int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& steps, std::vector<BGate>& bg) {
+  YH_TRY(to_bgates(nbits, ops, nops, bg));
+  return plan_gates(bg, nbits, structure_key(nbits, ops, nops), nops, steps);
+}
+
+// ---- density matrices: rho as a vector of 2n bits (row bits n..2n-1 above the
+// column bits 0..n-1, the row-major layout); a gate is U (x) conj(U) on the row and
+// column bits of its qubits, a channel sum_k K_k (x) conj(K_k) (density_matrix.rs
+// :161-224 computes the same maps as U rho U^dagger and sum_k K_k rho K_k^dagger).
+
+// the gate's full matrix on local qubits [targets..., controls...] (local bit k <->
+// qubit k): the target matrix where every control matches, identity elsewhere
+Mat with_controls(const Mat& g, const yh_op& op) {
+  const int nt = op.nt, k = op.nt + op.nc, D = 1 << k, dt = 1 << nt;
+  Mat F(size_t(D) * D, cd{0, 0});
+  int want = 0;
+  for (int c = 0; c < op.nc; ++c) want |= (op.ctrl_vals[c] ? 1 : 0) << c;
+  for (int i = 0; i < D; ++i) {
+    if ((i >> nt) != want) {
+      F[size_t(i) * D + i] = cd{1, 0};
+      continue;
+    }
+    for (int j = 0; j < dt; ++j) F[size_t(i) * D + ((i >> nt) << nt | j)] = g[size_t(i & (dt - 1)) * dt + j];
+  }
+  return F;
+}
+
+// superoperator of the maps rho -> M rho M^dagger summed over `ms` (each D x D on
+// the local qubits), on local bits [row bits..., column bits...]
+Mat superop(const std::vector<Mat>& ms, int k) {
+  const int D = 1 << k, S = D * D;
+  Mat out(size_t(S) * S, cd{0, 0});
+  for (const Mat& m : ms)
+    for (int i = 0; i < S; ++i)
+      for (int j = 0; j < S; ++j) {
+        const cd a = m[size_t(i & (D - 1)) * D + (j & (D - 1))];
+        const cd b = cconj(m[size_t(i >> k) * D + (j >> k)]);
+        out[size_t(i) * S + j] = out[size_t(i) * S + j] + a * b;
+      }
+  return out;
+}
+
+int to_bgates_dm(int n, const yh_op* ops, size_t nops, std::vector<BGate>& out) {
+  for (size_t e = 0; e < nops; ++e) {
+    const yh_op& op = ops[e];
+    std::vector<Mat> ms;
+    std::vector<int> q;
+    if (op.elem == YH_ELEM_GATE) {
+      Mat g;
+      const int d = gate_matrix(op, g);
+      if (d != (1 << op.nt)) return fail(YH_ERR_MATRIX_SIZE, "gate matrix size does not match targets");
+      for (int k = 0; k < op.nt; ++k) q.push_back(op.targets[k]);
+      for (int k = 0; k < op.nc; ++k) q.push_back(op.ctrls[k]);
+      if (2 * int(q.size()) <= kBQ) ms.push_back(with_controls(g, op));
+    } else if (op.elem == YH_ELEM_CHANNEL) {
+      int d = 0;
+      const int st = kraus_operators(op, ms, d);
+      if (st != YH_OK) return fail(st, "invalid channel parameters");
+      for (int k = 0; k < op.nt; ++k) q.push_back(op.targets[k]);
+      if (2 * int(q.size()) > kBQ) ms.clear();
+    } else {
+      continue;
+    }
+    BGate g;
+    g.op_index = e;
+    const int k = int(q.size());
+    g.nt = 2 * k;
+    for (int j = 0; j < k; ++j) g.tb.push_back(2 * n - 1 - q[j]);
+    for (int j = 0; j < k; ++j) g.tb.push_back(n - 1 - q[j]);
+    for (int b : g.tb) g.mask |= uint64_t(1) << b;
+    g.blockable = !ms.empty();
+    if (g.blockable) g.m = superop(ms, k);
+    out.push_back(std::move(g));
+  }
   return YH_OK;
 }
 
@@ -725,13 +803,12 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
 
 }  // namespace
 
+// Launches a block plan on a state of `nbits` bits; steps that are not blocks go
+// to `single` (the gate's own path).
 template <class R>
-int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops) {
+int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep>& steps,
+                   const std::vector<BGate>& bg, const std::function<int(const BGate&)>& single) {
   using V = typename Cplx<R>::V;
-  if (nbits < kBQ) return apply_gates<R>(w, state, nbits, ops, nops);
-  std::vector<BStep> steps;
-  std::vector<BGate> bg;
-  YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
   // persistent grids: as many workgroups as the CUs hold at once
   static int resident[2][2] = {{0, 0}, {0, 0}};
   int(&res)[2] = resident[sizeof(R) == 8 ? 0 : 1];
@@ -763,13 +840,7 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
   size_t off = 0;
   for (const PassItem& it : items) {
     if (it.single) {
-      const BGate& g = bg[steps[it.step].gate];
-      yh_op one = ops[g.op_index];
-      if (one.kind == YH_GATE_X && one.nt > 1) {
-        one.targets[0] = one.targets[g.sub];
-        one.nt = 1;
-      }
-      YH_TRY(apply_gates<R>(w, state, nbits, &one, 1));
+      YH_TRY(single(bg[steps[it.step].gate]));
       continue;
     }
     if (it.blocks.size() == 1) {
@@ -831,6 +902,36 @@ int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t 
     off += sz;
   }
   return w.sync();
+}
+
+template <class R>
+int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops) {
+  if (nbits < kBQ) return apply_gates<R>(w, state, nbits, ops, nops);
+  std::vector<BStep> steps;
+  std::vector<BGate> bg;
+  YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
+  return run_block_plan<R>(w, state, nbits, steps, bg, [&](const BGate& g) {
+    yh_op one = ops[g.op_index];
+    if (one.kind == YH_GATE_X && one.nt > 1) {
+      one.targets[0] = one.targets[g.sub];
+      one.nt = 1;
+    }
+    return apply_gates<R>(w, state, nbits, &one, 1);
+  });
+}
+
+int apply_dm_blocks(Workspace& w, void* rho, int n, const yh_op* ops, size_t nops,
+                    const std::function<int(size_t)>& single_op) {
+  std::vector<BGate> bg;
+  YH_TRY(to_bgates_dm(n, ops, nops, bg));
+  if (bg.empty()) return YH_OK;
+  if (2 * n < kBQ) {  // too few bits for a block: every element on its own path
+    for (const BGate& g : bg) YH_TRY(single_op(g.op_index));
+    return YH_OK;
+  }
+  std::vector<BStep> steps;
+  YH_TRY(plan_gates(bg, 2 * n, structure_key(2 * n, ops, nops) ^ 0x9e3779b97f4a7c15ull, nops, steps));
+  return run_block_plan<double>(w, rho, 2 * n, steps, bg, [&](const BGate& g) { return single_op(g.op_index); });
 }
 
 template int apply_blocks<double>(Workspace&, void*, int, const yh_op*, size_t);

@@ -1,6 +1,7 @@
 // runtime.h — device buffers and the per-handle workspace (one HIP stream each).
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "common.h"
@@ -77,6 +78,11 @@ template <class R>
 int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]);
 int block_plan_bits(int nbits, const yh_op* ops, size_t nops, int32_t* bits, size_t cap, size_t* nsteps);
+// Density matrix (2n-bit vector, row bits above column bits) through the block
+// path: gates and channels as superoperators; an element on more than 5 bits goes
+// to single_op(op index).
+int apply_dm_blocks(Workspace& w, void* rho, int n, const yh_op* ops, size_t nops,
+                    const std::function<int(size_t)>& single_op);
 
 }  // namespace yh
 

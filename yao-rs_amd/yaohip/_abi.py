@@ -155,6 +155,7 @@ SIGNATURES = [
     ("yh_dm_download", c_int, [c_void_p, P(c_double), c_size_t]),
     ("yh_dm_apply", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
     ("yh_dm_apply_fused", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
+    ("yh_dm_apply_blocks", c_int, [c_void_p, c_int, P(YhOp), c_size_t]),
     ("yh_dm_trace", c_int, [c_void_p, P(c_double)]),
     ("yh_dm_purity", c_int, [c_void_p, P(c_double)]),
     ("yh_dm_partial_tr", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_void_p)]),

@@ -258,9 +258,14 @@ class DensityMatrix:
         return A.deinterleave(buf)
 
     def apply(self, circuit: Circuit, fused=False):
-        """Register::apply for DensityMatrix (density_matrix.rs:244-259)."""
+        """Register::apply for DensityMatrix (density_matrix.rs:244-259). fused=True:
+        each gate with its following 1-qubit channels in one pass; fused="blocks":
+        gates and channels as superoperators fused into 5-bit MFMA blocks."""
         ops = circuit.ops
-        fn = A.lib().yh_dm_apply_fused if fused else A.lib().yh_dm_apply
+        if fused == "blocks":
+            fn = A.lib().yh_dm_apply_blocks
+        else:
+            fn = A.lib().yh_dm_apply_fused if fused else A.lib().yh_dm_apply
         A.check(fn(self._h, circuit.nbits, ops.arr, ops.n))
 
     def trace(self):
