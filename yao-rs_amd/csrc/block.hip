@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256, 3) void k_block5(typename Cplx<R>::V* __restri
 }
 
 // ---- several blocks per HBM pass: an LDS tile of 2^12 amplitudes ----
-// The tile holds memory bits 0..4 (runs of 32 consecutive amplitudes) plus 7 more;
-// every block of the pass has its 5 bits among them. One 512-thread workgroup per
+// The tile holds memory bits 0..3 (runs of 16 consecutive amplitudes, 256 B in
+// complex128) plus 8 more; every block of the pass has its 5 bits among them. One 512-thread workgroup per
 // CU keeps the tile and the A values of all the pass's blocks in LDS (the A values
 // once per launch). Per tile: the amplitudes, loaded into registers during the
 // previous tile, go to LDS; the next tile's loads are issued; each block is the
@@ -196,13 +196,14 @@ __global__ __launch_bounds__(256, 3) void k_block5(typename Cplx<R>::V* __restri
 // Ui times the negated imaginary parts, which rounds exactly as k_block5 does.
 constexpr int kTT = 12;                // tile bits
 constexpr int kTN = 1 << kTT;          // 4096 amplitudes
-constexpr int kTHigh = kTT - kBQ;      // 7 tile bits above the low 5
+constexpr int kTLow = 4;               // memory bits 0..3 are always tile bits
+constexpr int kTHigh = kTT - kTLow;    // 8 tile bits chosen per pass
 constexpr int kTThreads = 512;         // 8 waves: one 16-column step each per block
 constexpr int kPassMax = 5;            // blocks per pass
 constexpr int kCFrag = 32 * 64;        // compact A values of one block
 
 struct TBlockArgs {
-  uint8_t tbit[kTT];          // memory bit of tile position k, ascending (0..4 first)
+  uint8_t tbit[kTT];          // memory bit of tile position k, ascending (0..3 first)
   uint8_t q[kPassMax][kBQ];   // tile position of row bit j of block b
   uint8_t qs[kPassMax][kBQ];  // the same positions, ascending
   int32_t nblk;
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
     uint64_t o = 0;
 #pragma unroll
     for (int k = 0; k < kTHigh; ++k)
-      if ((tid >> k) & 1) o |= uint64_t(1) << a.tbit[kBQ + k];
+      if ((tid >> k) & 1) o |= uint64_t(1) << a.tbit[kTLow + k];
     hoff[tid] = o;
   }
   if (tid < kPassMax * kBQ) {
@@ -244,7 +245,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int p = tid + kTThreads * k;
-    const V x = s[base + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]];
+    const V x = s[base + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]];
     vx[k] = x.x;
     vy[k] = x.y;
   }
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int p = tid + kTThreads * k;
-      const V x = s[bn + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]];
+      const V x = s[bn + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]];
       vx[k] = x.x;
       vy[k] = x.y;
     }
@@ -321,7 +322,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int p = tid + kTThreads * k;
-      s[base + uint64_t(p & (kBD - 1)) + hoff[p >> kBQ]] = lds[tpad(p)];
+      s[base + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]] = lds[tpad(p)];
     }
     lds_barrier();  // the next tile refills LDS
     if (!more) break;
@@ -743,7 +744,7 @@ void to_compact(const std::vector<cd>& U, R* o) {
 // Launch order of a block plan: gates run alone stay where they are; each run of
 // consecutive blocks is split into passes of k_tblocks. A pass starts at the first
 // block not yet placed and takes, in order, every later block whose bits keep the
-// tile (bits 0..4 plus the pass's block bits) within 12 bits and that shares no bit
+// tile (bits 0..3 plus the pass's block bits) within 12 bits and that shares no bit
 // with a block skipped before it, so blocks only move past blocks on disjoint bits.
 struct PassItem {
   bool single = false;
@@ -775,7 +776,7 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
     for (size_t k = i; k < j; ++k) rem.push_back(k);
     while (!rem.empty()) {
       PassItem it;
-      uint64_t cur = (uint64_t(1) << kBQ) - 1, blocked = 0;
+      uint64_t cur = (uint64_t(1) << kTLow) - 1, blocked = 0;
       std::vector<size_t> left;
       for (size_t k : rem) {
         const uint64_t bm = step_mask(steps[k]);
