@@ -192,6 +192,9 @@ def main():
             "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
             "traffic": TRAFFIC28_BYTES["tblock"] if n == 28 else None, "kernel": "tblock",
             "blocks_per_launch": round(bpl, 3), "flops_per_launch": int(bpl * flops_block),
+            # the kernel forms each complex product from three real ones (Gauss), so
+            # the MFMA units execute 3/4 of the algorithmic flops
+            "mfma_flops_executed_per_launch": int(bpl * flops_block * 3 // 4),
             "avg_launch_us": round(t["avg_us"], 2),
             "hbm_gbps": round(2 * 16 * (1 << n) / (t["avg_us"] * 1e-6) / 1e9, 1)}
     dom = max((k for k in ("fused", "tblock") if k in kstats), key=lambda k: kstats[k]["total_ms"])

@@ -191,9 +191,10 @@ __global__ __launch_bounds__(256, 3) void k_block5(typename Cplx<R>::V* __restri
 // same real-split MFMA product as k_block5 on the tile viewed as a 32 x 128 matrix
 // (one 16-column step per wave), with a barrier between blocks; the tile is stored.
 // Barriers inside the loop order LDS only, so the next tile's loads stay in flight
-// while the MFMAs run. U's real-split matrix [[Ur, -Ui], [Ui, Ur]] only holds Ur
-// and Ui, so 32 A values per lane are kept, not 64: the -Ui products are taken as
-// Ui times the negated imaginary parts, which rounds exactly as k_block5 does.
+// while the MFMAs run. The complex product U S is formed from three real products
+// (Gauss): Ur Sr, Ui Si and (Ur + Ui)(Sr + Si), 48 MFMAs per 16-column step instead
+// of the 64 of the real-split form; the A values are Ur and Ui (32 per lane).
+// Rounding differs from k_block5's at the level of a few ulps of |U||S|.
 constexpr int kTT = 12;                // tile bits
 constexpr int kTN = 1 << kTT;          // 4096 amplitudes
 constexpr int kTLow = 4;               // memory bits 0..3 are always tile bits
@@ -291,30 +292,31 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       V x[8];
 #pragma unroll
       for (int m = 0; m < 8; ++m) x[m] = lds[tpad(cb | in_lane | roff(4 * m))];
-      typename M::acc_t acc[4];
+      // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
+      // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2)
+      typename M::acc_t p1[2], p2[2], p3[2];
 #pragma unroll
-      for (int rt = 0; rt < 4; ++rt) acc[rt] = typename M::acc_t{0, 0, 0, 0};
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const R br = x[ks].x;
-        acc[0] = M::mfma(ur[ks], br, acc[0]);
-        acc[1] = M::mfma(ur[8 + ks], br, acc[1]);
-        acc[2] = M::mfma(ui[ks], br, acc[2]);
-        acc[3] = M::mfma(ui[8 + ks], br, acc[3]);
+      for (int rt = 0; rt < 2; ++rt) {
+        p1[rt] = typename M::acc_t{0, 0, 0, 0};
+        p2[rt] = typename M::acc_t{0, 0, 0, 0};
+        p3[rt] = typename M::acc_t{0, 0, 0, 0};
       }
 #pragma unroll
       for (int ks = 0; ks < 8; ++ks) {
-        const R bi = x[ks].y, nbi = -x[ks].y;
-        acc[0] = M::mfma(ui[ks], nbi, acc[0]);
-        acc[1] = M::mfma(ui[8 + ks], nbi, acc[1]);
-        acc[2] = M::mfma(ur[ks], bi, acc[2]);
-        acc[3] = M::mfma(ur[8 + ks], bi, acc[3]);
+        const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
+          p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
+          p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+        }
       }
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         V o;
-        o.x = acc[m >> 2][m & 3];
-        o.y = acc[2 + (m >> 2)][m & 3];
+        const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
+        o.x = a1 - a2;
+        o.y = p3[m >> 2][m & 3] - a1 - a2;
         lds[tpad(cb | out_lane | roff(16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0))))] = o;
       }
       lds_barrier();
