@@ -30,12 +30,12 @@ HBM_PEAK_GBPS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 # HBM bytes per launch on the 28-qubit state, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 11 of k_tblocks,
-# 20 of k_block5), each scaled by the factor a plain copy of known size gives for
+# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 20 of k_tblocks,
+# 3 of k_block5), each scaled by the factor a plain copy of known size gives for
 # the same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
 # profiles/k_block5_brickwork28_pmc.md.
-TRAFFIC28_BYTES = {"tblock": int((2 * 2097575 + 4194304) * 1024),
-                   "fused": int((2 * 2097501 + 4194304) * 1024)}
+TRAFFIC28_BYTES = {"tblock": int((2 * 2097560 + 4194304) * 1024),
+                   "fused": int((2 * 2097515 + 4194304) * 1024)}
 NBITS = 28
 DEPTH = 20
 SEED = 20
