@@ -7,6 +7,7 @@ from . import _abi
 from .circuit import (Annotation, Circuit, CircuitError, Gate, NoiseChannel, Op, OperatorPolynomial,
                       OperatorString, PositionedChannel, PositionedGate, channel, control, label, lower,
                       put, tfim_hamiltonian)
+from .circuit_json import CircuitJsonError, circuit_from_json, circuit_to_json
 from .register import (ArrayReg, DensityMatrix, PostProcess, apply, apply_inplace, channel_kraus,
                        expect_arrayreg, expect_dm, expect_grad, gate_matrix, measure_with_postprocess, probs,
                        read_state, read_state_header, sample, write_state)
@@ -14,9 +15,9 @@ from .register import (ArrayReg, DensityMatrix, PostProcess, apply, apply_inplac
 YhError = _abi.YhError
 
 __all__ = [
-    "Annotation", "ArrayReg", "Circuit", "CircuitError", "DensityMatrix", "Gate", "NoiseChannel", "Op",
+    "Annotation", "ArrayReg", "Circuit", "CircuitError", "CircuitJsonError", "DensityMatrix", "Gate", "NoiseChannel", "Op",
     "OperatorPolynomial", "OperatorString", "PositionedChannel", "PositionedGate", "PostProcess", "YhError",
-    "apply", "apply_inplace", "channel", "channel_kraus", "control", "expect_arrayreg", "expect_dm", "expect_grad",
+    "apply", "apply_inplace", "channel", "channel_kraus", "circuit_from_json", "circuit_to_json", "control", "expect_arrayreg", "expect_dm", "expect_grad",
     "gate_matrix", "label", "lower", "measure_with_postprocess", "probs", "put", "read_state", "read_state_header",
     "sample", "tfim_hamiltonian", "write_state",
 ]
