@@ -258,6 +258,7 @@ def main():
                             "dtype": "f32 amplitudes and FP32 MFMA, block matrices fused in f64 on the host"}
         log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
         del reg64, reg64b, got64, ref
+        out["qft12"] = bench_qft12(log)
         out["vqe_batched"] = bench_vqe(log)
         out["noisy_dm"] = bench_dm(log)
         out["surrogate"] = bench_surrogate(log)
@@ -266,6 +267,55 @@ def main():
         del reg
         out["cpu_baseline"] = cpu_baseline(els, n)
     print(json.dumps(out), flush=True)
+
+
+def bench_qft12(log, reps=200):
+    """12-qubit QFT with SWAPs (84 gates) on a seeded N(0,1) state, then Z(0)Z(1)
+    and 1,024 shots from fixed uniforms: wall time per circuit application (the
+    state, 64 KiB, is latency-bound work), gate by gate and as fused blocks, plus
+    the expectation and sampling calls. Parity is tested against the oracle. At 12
+    qubits the whole state is one LDS tile, so the tile path (max_fuse_qubits 1..4)
+    runs the circuit in one launch of one workgroup."""
+    import yaohip
+    from yaohip.circuit import Circuit, OperatorPolynomial, OperatorString, Op
+    from fixture_circuits import qft_with_swaps
+    n = 12
+    rng = np.random.default_rng(2024)
+    psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+    psi /= np.linalg.norm(psi)
+    circ = Circuit.qubits(n, qft_with_swaps(n))
+    reg = yaohip.ArrayReg.from_vec(n, psi)
+    out = {"nbits": n, "gates": len(circ.elements)}
+    for fused, key in ((0, "gate_by_gate"), (5, "fused_blocks"), (4, "lds_tile")):
+        yaohip.apply_inplace(circ, reg, fused)
+        reg.norm()  # drains the stream
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            yaohip.apply_inplace(circ, reg, fused)
+        reg.norm()
+        out[key + "_us_per_circuit"] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+    # device time of the one-launch tile pass (HIP events around the launch)
+    from yaohip import _abi as A
+    A.check(A.lib().yh_reg_timing_enable(reg._h, 1))
+    for _ in range(reps):
+        yaohip.apply_inplace(circ, reg, 4)
+    c, ms = ctypes.c_uint64(), ctypes.c_double()
+    A.check(A.lib().yh_reg_timing_read(reg._h, A.K_TILE, ctypes.byref(c), ctypes.byref(ms)))
+    A.check(A.lib().yh_reg_timing_enable(reg._h, 0))
+    out["lds_tile_launches_per_circuit"] = round(c.value / reps, 2)
+    out["lds_tile_device_us_per_circuit"] = round(1e3 * ms.value / reps, 1)
+    zz = OperatorPolynomial([1.0], [OperatorString([(0, Op.Z), (1, Op.Z)])])
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        yaohip.expect_arrayreg(reg, zz)
+    out["expect_zz_us"] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+    u = np.random.default_rng(1024).random(1024)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        yaohip.sample(reg, None, u)
+    out["shots_1024_us"] = round((time.perf_counter() - t0) / reps * 1e6, 1)
+    log(f"qft12: {out}")
+    return out
 
 
 def bench_vqe(log, count=4096, plain_sample=256):

@@ -236,13 +236,16 @@ def test_measure_errors(gpu):
     assert e.value.status == 6
 
 
-def test_qft12_config(gpu):
-    """12-qubit QFT with SWAPs on a seeded N(0,1) state: amplitudes, Z(0)Z(1), 1024 shots."""
+@pytest.mark.parametrize("fused", [0, 4, 5])
+def test_qft12_config(gpu, fused):
+    """12-qubit QFT with SWAPs on a seeded N(0,1) state: amplitudes, Z(0)Z(1), 1024 shots,
+    gate by gate, as one LDS tile pass (the whole 12-qubit state is one tile) and as
+    fused blocks."""
     n = 12
     psi = seeded_state(n, 2024)
     els = qft_with_swaps(n)
     reg = gpu.ArrayReg.from_vec(n, psi)
-    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg, fused)
     want = O.apply(n, psi, lower(els))
     assert rel_err(reg.state_vec(), want) < TOL
     zz = OperatorPolynomial([1.0], [OperatorString([(0, Op.Z), (1, Op.Z)])])

@@ -8,6 +8,7 @@ usage: python yao-rs_amd/prof_workloads.py WORKLOAD [nbits] [depth]
   copy       one device-to-device copy of a 2^nbits complex128 state (the
              reference kernel for scaling FETCH_SIZE / WRITE_SIZE)
   vqe        one batch of 4096 ansatz instances (n in 10..20, depth 4..8), complex64
+  vqe128     the same batch in complex128
   dm         noisy brickwork on a density matrix (fused gate+channel passes)
 Each workload prints one line naming itself before it touches the GPU.
 """
@@ -43,10 +44,10 @@ def main():
         for _ in range(3):
             A.check(A.lib().yh_reg_copy(b._h, a._h))
         print(f"[prof] copy of {16 << n} bytes x3 done", flush=True)
-    elif w == "vqe":
+    elif w in ("vqe", "vqe128"):
         from yaohip.vqe import VqeBatch, random_instances
         inst = random_instances(4096, seed=4)
-        b = VqeBatch(4096, 20, "c64")
+        b = VqeBatch(4096, 20, "c64" if w == "vqe" else "c128")
         e = b.energies(inst)
         t0 = time.perf_counter()
         e = b.energies(inst)
