@@ -364,7 +364,7 @@ def bench_vqe(log, count=4096, plain_sample=256):
 
 
 def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=65536, keep_frac=0.01,
-                    epochs=6, batch=1024, lr=1e-3, mae_sample=4096, plain_sample=64):
+                    schedule=((40, 1e-3), (10, 3e-4)), batch=4096, mae_sample=4096, plain_sample=64):
     """Learned surrogate: 288 angles (16 qubits, depth 6, Rz-Ry-Rz per qubit per
     layer, CZ chain; angles uniform in [-pi/4, pi/4]) -> TFIM energy (J = h = 1).
     MLP 576 trig features -> 4 x 256 GELU -> 1, BF16 MFMA with FP32 accumulation,
@@ -388,11 +388,13 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
     res["datagen_evals_per_s"] = round(train_pairs / tg, 1)
     log(f"surrogate: generated {train_pairs} energies in {tg:.2f} s")
     t0 = time.perf_counter()
-    losses = s.fit(train, e, epochs=epochs, batch=batch, lr=lr, seed=7)
+    losses = s.fit(train, e, batch=batch, seed=7, schedule=list(schedule))
     tt = time.perf_counter() - t0
     del train, e
+    epochs = sum(ep for ep, _ in schedule)
     fwd_flops = 2 * (2 * s.n_angles * 256 + 3 * 256 * 256 + 256)
-    res["train"] = {"epochs": epochs, "batch": batch, "lr": lr, "seconds": round(tt, 3),
+    res["train"] = {"epochs": epochs, "batch": batch, "schedule": [list(p) for p in schedule],
+                    "seconds": round(tt, 3),
                     "epoch_loss_standardised": [round(float(v), 5) for v in losses],
                     "samples_per_s": round(train_pairs * epochs / tt, 1),
                     "tflops_algorithmic": round(3 * fwd_flops * train_pairs * epochs / tt / 1e12, 2)}

@@ -117,12 +117,17 @@ class SurrogateScreen:
     def sample_angles(self, count, seed, lo=-math.pi / 4, hi=math.pi / 4):
         return np.random.default_rng(seed).uniform(lo, hi, size=(count, self.n_angles))
 
-    def fit(self, angles, energies, epochs=4, batch=1024, lr=1e-3, seed=0):
-        """Standardises the targets and trains; returns the per-epoch losses (standardised units)."""
+    def fit(self, angles, energies, epochs=4, batch=1024, lr=1e-3, seed=0, schedule=None):
+        """Standardises the targets and trains; returns the per-epoch losses (standardised units).
+        schedule: optional [(epochs, lr), ...] phases run in order (Adam's state carries
+        over, so a later phase is a learning-rate drop); it replaces epochs and lr."""
         e = np.asarray(energies, dtype=np.float64)
         self.mu, self.sigma = float(e.mean()), float(e.std() or 1.0)
         y = (e - self.mu) / self.sigma
-        return self.mlp.train(_f32(angles), y, batch, epochs, lr, seed)
+        x = _f32(angles)
+        phases = schedule if schedule is not None else [(epochs, lr)]
+        losses = [self.mlp.train(x, y, batch, ep, rate, seed + k) for k, (ep, rate) in enumerate(phases)]
+        return np.concatenate(losses)
 
     def predict(self, angles):
         return self.mlp.predict(_f32(angles)).astype(np.float64) * self.sigma + self.mu
