@@ -321,12 +321,13 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       }
       lds_barrier();
     }
+    // each thread stores, then (next iteration) refills, the same LDS positions, so
+    // no barrier is needed between the two
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
       const int p = tid + kTThreads * k;
       s[base + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]] = lds[tpad(p)];
     }
-    lds_barrier();  // the next tile refills LDS
     if (!more) break;
     t = tn;
     base = bn;
