@@ -9,6 +9,8 @@ usage: python yao-rs_amd/prof_workloads.py WORKLOAD [nbits] [depth]
              reference kernel for scaling FETCH_SIZE / WRITE_SIZE)
   vqe        one batch of 4096 ansatz instances (n in 10..20, depth 4..8), complex64
   vqe128     the same batch in complex128
+  k1q_bits   Rx on every qubit of an nbits complex128 state (4 gates per qubit),
+             printing k_1q's average launch time per target bit (HIP events)
   dm         noisy brickwork on a density matrix (fused gate+channel passes)
 Each workload prints one line naming itself before it touches the GPU.
 """
@@ -52,6 +54,23 @@ def main():
         t0 = time.perf_counter()
         e = b.energies(inst)
         print(f"[prof] vqe batch {time.perf_counter() - t0:.4f} s mean E {e.mean():.6f}", flush=True)
+    elif w == "k1q_bits":
+        import ctypes
+        from yaohip import _abi as A
+        from yaohip.circuit import Gate, put
+        reg = yaohip.ArrayReg.zero_state(n)
+        L = A.lib()
+        for loc in range(n):
+            circ = Circuit.qubits(n, [put([loc], Gate.Rx(0.3))] * 4)
+            yaohip.apply_inplace(circ, reg)
+            A.check(L.yh_reg_timing_enable(reg._h, 1))
+            yaohip.apply_inplace(circ, reg)
+            c, ms = ctypes.c_uint64(), ctypes.c_double()
+            A.check(L.yh_reg_timing_read(reg._h, A.K_1Q, ctypes.byref(c), ctypes.byref(ms)))
+            A.check(L.yh_reg_timing_enable(reg._h, 0))
+            us = 1e3 * ms.value / c.value
+            print(f"[prof] k_1q target bit {n - 1 - loc:2d}: {us:8.1f} us, "
+                  f"{2 * 16 * (1 << n) / (us * 1e-6) / 1e12:.2f} TB/s", flush=True)
     elif w == "dm":
         dm = yaohip.DensityMatrix.zero_state(n)
         dm.apply(Circuit.qubits(n, noisy_brickwork(n, depth, 3)), fused=True)
