@@ -174,29 +174,34 @@ def main():
     A.check(L.yh_copy_bandwidth(16 << n, 5, ctypes.byref(gb)))
     copy_gbps = gb.value  # device-to-device copy of a state-sized buffer, HIP events
     log(f"device copy of the state: {copy_gbps:.1f} GB/s")
-    flops_block = (1 << n) * 64 * 64 * 2 // 32
-    per_kernel = {}
-    if "fused" in kstats:
-        r = roofline_of(kstats, dom="fused")
-        r["traffic"] = TRAFFIC28_BYTES["fused"] if n == 28 else None
-        r["copy_gbps"] = round(copy_gbps, 1)
-        r["frac_of_copy"] = round(r["achieved"] / copy_gbps, 4)
-        per_kernel["k_block5"] = r
-    if "tblock" in kstats:
-        t = kstats["tblock"]
-        nb_tile = int(bplan[0]) * args.steps - kstats.get("fused", {}).get("launches", 0)
-        bpl = nb_tile / t["launches"]
-        tf = bpl * flops_block / (t["avg_us"] * 1e-6) / 1e12
-        per_kernel["k_tblocks"] = {
-            "bound": "mfma", "achieved": round(tf, 2), "peak": FP64_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(tf / FP64_MFMA_PEAK_TFLOPS, 4),
-            "traffic": TRAFFIC28_BYTES["tblock"] if n == 28 else None, "kernel": "tblock",
-            "blocks_per_launch": round(bpl, 3), "flops_per_launch": int(bpl * flops_block),
-            # the kernel forms each complex product from three real ones (Gauss), so
-            # the MFMA units execute 3/4 of the algorithmic flops
-            "mfma_flops_executed_per_launch": int(bpl * flops_block * 3 // 4),
-            "avg_launch_us": round(t["avg_us"], 2),
-            "hbm_gbps": round(2 * 16 * (1 << n) / (t["avg_us"] * 1e-6) / 1e9, 1)}
+    def block_rooflines(ks, amp_bytes, peak_tflops, steps):
+        """k_block5 (one block per launch) as HBM-bound, k_tblocks (several per
+        launch) as MFMA-bound, from the launches' HIP-event averages."""
+        flops_block = (1 << n) * 64 * 64 * 2 // 32
+        traffic = TRAFFIC28_BYTES if (n == 28 and amp_bytes == 16) else {}
+        per = {}
+        if "fused" in ks:
+            r = roofline_of(ks, amp_bytes=amp_bytes, dom="fused")
+            r["traffic"] = traffic.get("fused")
+            r["copy_gbps"] = round(copy_gbps, 1)
+            r["frac_of_copy"] = round(r["achieved"] / copy_gbps, 4)
+            per["k_block5"] = r
+        if "tblock" in ks:
+            t = ks["tblock"]
+            bpl = (int(bplan[0]) * steps - ks.get("fused", {}).get("launches", 0)) / t["launches"]
+            tf = bpl * flops_block / (t["avg_us"] * 1e-6) / 1e12
+            per["k_tblocks"] = {
+                "bound": "mfma", "achieved": round(tf, 2), "peak": peak_tflops, "unit": "TFLOP/s",
+                "frac": round(tf / peak_tflops, 4), "traffic": traffic.get("tblock"), "kernel": "tblock",
+                "blocks_per_launch": round(bpl, 3), "flops_per_launch": int(bpl * flops_block),
+                # the kernel forms each complex product from three real ones (Gauss),
+                # so the MFMA units execute 3/4 of the algorithmic flops
+                "mfma_flops_executed_per_launch": int(bpl * flops_block * 3 // 4),
+                "avg_launch_us": round(t["avg_us"], 2),
+                "hbm_gbps": round(2 * amp_bytes * (1 << n) / (t["avg_us"] * 1e-6) / 1e9, 1)}
+        return per
+
+    per_kernel = block_rooflines(kstats, 16, FP64_MFMA_PEAK_TFLOPS, args.steps)
     dom = max((k for k in ("fused", "tblock") if k in kstats), key=lambda k: kstats[k]["total_ms"])
     roof = dict(per_kernel["k_block5" if dom == "fused" else "k_tblocks"])
     roof["copy_gbps"] = round(copy_gbps, 1)
@@ -253,7 +258,7 @@ def main():
         out["complex64"] = {"value": round(ngates * steps64 / dt64, 2), "unit": "gate-apps/s",
                             "ms_per_step": round(1e3 * dt64 / steps64, 3),
                             "max_rel_err_vs_fp64": err64, "fidelity_vs_fp64": fid64,
-                            "roofline": roofline_of(k64, amp_bytes=8),
+                            "roofline_kernels": block_rooflines(k64, 8, FP32_MFMA_PEAK_TFLOPS, steps64),
                             "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS, int(bplan[0]), steps64),
                             "dtype": "f32 amplitudes and FP32 MFMA, block matrices fused in f64 on the host"}
         log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
