@@ -162,6 +162,28 @@ def test_block_plan_covers_every_gate():
     assert int(out[2]) == 1 and int(out[1]) == 10
 
 
+def test_block_plan_bits():
+    """yh_block_plan lists each block's 5 distinct memory bits, in plan order, and
+    -1 rows for gates run alone; every gate's bits lie inside some block."""
+    from fixture_circuits import brickwork
+    n = 28
+    els = brickwork(n, 4, 5)
+    wide = PositionedGate(Gate.Custom(np.eye(8), False), [0, 1, 2], [3, 4, 5], [True, True, False])
+    ol = lower(els + [wide])
+    ns = ctypes.c_size_t()
+    A.check(A.lib().yh_block_plan(n, ol.arr, ol.n, None, 0, ctypes.byref(ns)))
+    bits = np.zeros(5 * ns.value, dtype=np.int32)
+    A.check(A.lib().yh_block_plan(n, ol.arr, ol.n, A.i32ptr(bits), ns.value, ctypes.byref(ns)))
+    rows = bits.reshape(-1, 5)
+    singles = [r for r in rows if (r == -1).all()]
+    blocks = [set(int(b) for b in r) for r in rows if (r >= 0).all()]
+    assert len(singles) == 1 and len(singles) + len(blocks) == len(rows)
+    assert all(len(b) == 5 and max(b) < n for b in blocks)
+    for e in els:  # memory bit of loc l is n - 1 - l
+        g = {n - 1 - l for l in list(e.target_locs) + list(e.control_locs)}
+        assert any(g <= b for b in blocks)
+
+
 def test_circuit_num_params_matches_python_mirror():
     from test_ad_oracle import random_param_circuit
     els = random_param_circuit(5, 60, 3)

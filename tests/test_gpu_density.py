@@ -67,7 +67,7 @@ def random_noisy(n, ngates, seed):
 
 
 @pytest.mark.parametrize("fused", [False, True, "blocks"])
-@pytest.mark.parametrize("n,seed", [(1, 0), (3, 1), (5, 2), (7, 3)])
+@pytest.mark.parametrize("n,seed", [(1, 0), (3, 1), (5, 2), (7, 3), (9, 4)])
 def test_random_noisy_circuits_vs_oracle(gpu, n, seed, fused):
     els = random_noisy(n, 40, 100 + seed)
     psi = seeded_state(n, seed)
