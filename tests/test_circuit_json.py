@@ -78,6 +78,20 @@ def test_round_trip_keeps_every_element(name):
                    (b.gate.kind, b.gate.params, b.target_locs, b.control_locs, b.control_configs)
 
 
+@pytest.mark.parametrize("name", ["visualization-example-qft", "visualization-example-basic"])
+def test_visualization_examples_parse_and_round_trip(name):
+    """The circuits of the reference's visualisation page (visualization/example-*.json,
+    copied as *.circuit): they parse, round-trip, and the oracle keeps their norm."""
+    with open(os.path.join(EXAMPLES, name + ".circuit")) as f:
+        text = f.read()
+    c = circuit_from_json(text)
+    assert json.loads(circuit_to_json(c)) == json.loads(circuit_to_json(circuit_from_json(circuit_to_json(c))))
+    n = c.nbits
+    z = np.zeros(1 << n, dtype=complex)
+    z[0] = 1
+    assert abs(np.linalg.norm(O.apply(n, z, lower(c.elements))) - 1) < 1e-14
+
+
 def test_structure_and_optional_fields():
     c = Circuit.qubits(2, [put([0], Gate.H), control([0], [1], Gate.X), label(1, "test"),
                            put([1], Gate.Rx(0.5)), channel([0], NoiseChannel.BitFlip(0.1))])
