@@ -184,17 +184,24 @@ __global__ __launch_bounds__(256, 3) void k_block5(typename Cplx<R>::V* __restri
 
 // ---- several blocks per HBM pass: an LDS tile of 2^12 amplitudes ----
 // The tile holds memory bits 0..3 (runs of 16 consecutive amplitudes, 256 B in
-// complex128) plus 8 more; every block of the pass has its 5 bits among them. One 512-thread workgroup per
-// CU keeps the tile and the A values of all the pass's blocks in LDS (the A values
-// once per launch). Per tile: the amplitudes, loaded into registers during the
-// previous tile, go to LDS; the next tile's loads are issued; each block is the
-// same real-split MFMA product as k_block5 on the tile viewed as a 32 x 128 matrix
-// (one 16-column step per wave), with a barrier between blocks; the tile is stored.
-// Barriers inside the loop order LDS only, so the next tile's loads stay in flight
-// while the MFMAs run. The complex product U S is formed from three real products
-// (Gauss): Ur Sr, Ui Si and (Ur + Ui)(Sr + Si), 48 MFMAs per 16-column step instead
-// of the 64 of the real-split form; the A values are Ur and Ui (32 per lane).
-// Rounding differs from k_block5's at the level of a few ulps of |U||S|.
+// complex128) plus 8 more; every block of the pass has its 5 bits among them. One
+// 512-thread workgroup per CU keeps the tile and the A values of all the pass's
+// blocks in LDS (the A values once per launch). Each block is an MFMA product on
+// the tile viewed as a 32 x 128 matrix (one 16-column step per wave), with a
+// barrier between blocks. The complex product U S is formed from three real
+// products (Gauss): Ur Sr, Ui Si and (Ur + Ui)(Sr + Si), 48 MFMAs per 16-column step
+// instead of the 64 of the real-split form; the A values are Ur and Ui (32 per
+// lane). Rounding differs from k_block5's at the level of a few ulps of |U||S|.
+// Moving a tile in and out:
+//  - DIN: the first block's operands go from HBM straight into registers, in its
+//    MFMA layout (loaded during the previous tile), and its results fill LDS;
+//    otherwise the tile is loaded into registers during the previous tile, in
+//    thread order, and written to LDS before the first block.
+//  - DOUT: the last block's results go from registers straight to HBM; otherwise
+//    the tile is stored from LDS after the last block.
+// The host sets DIN / DOUT for a block whose rows avoid memory bits 0..3: its 16
+// columns per row are then 16 consecutive amplitudes in memory. Barriers inside the
+// loop order LDS only, so the next tile's loads stay in flight while the MFMAs run.
 constexpr int kTT = 12;                // tile bits
 constexpr int kTN = 1 << kTT;          // 4096 amplitudes
 constexpr int kTLow = 4;               // memory bits 0..3 are always tile bits
@@ -213,7 +220,7 @@ struct TBlockArgs {
 
 __device__ __forceinline__ int tpad(int p) { return p + (p >> 4); }
 
-template <class R>
+template <class R, bool DIN, bool DOUT>
 __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* __restrict__ s, TBlockArgs a,
                                                         const R* __restrict__ frag) {
   using V = typename Cplx<R>::V;
@@ -239,32 +246,62 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   for (int e = tid; e < a.nblk * kCFrag; e += kTThreads) afr[e] = frag[e];
   __syncthreads();
   constexpr int kPer = kTN / kTThreads;
+  // memory offset (from the tile's base) of tile position p
+  auto mem = [&](int p) { return uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]; };
+  // geometry of block b for this lane: row offsets and the column's tile position
+  struct Geo {
+    int q[kBQ];
+    int cb, in_lane, out_lane;
+  };
+  auto roff = [](const Geo& g, int r) {
+    int o = 0;
+#pragma unroll
+    for (int j = 0; j < kBQ; ++j) o |= ((r >> j) & 1) << g.q[j];
+    return o;
+  };
+  auto geo = [&](int b, Geo& g) {
+    int qs[kBQ];
+#pragma unroll
+    for (int j = 0; j < kBQ; ++j) {
+      g.q[j] = bq[b][j];
+      qs[j] = bq[b][kBQ + j];
+    }
+    g.in_lane = roff(g, grp);
+    g.out_lane = roff(g, M::drow(lane, 0));
+    int cb = wave * kCols + col;  // this lane's column, then its tile position
+#pragma unroll
+    for (int j = 0; j < kBQ; ++j) cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
+    g.cb = cb;
+  };
   uint64_t t = blockIdx.x;
   if (t >= a.ntiles) return;  // uniform
   uint64_t base = insert_zeros(t, a.tbit, kTT);
-  R vx[kPer], vy[kPer];  // scalar arrays: kept in registers across the loop
+  // DIN: the first block's operands of the next tile; else the next tile in thread order
+  R vx[8], vy[8];
+  static_assert(kPer == 8, "eight amplitudes per thread");
+  Geo g0;
+  geo(0, g0);
+  auto load_next = [&](uint64_t bb) {
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int p = tid + kTThreads * k;
-    const V x = s[base + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]];
-    vx[k] = x.x;
-    vy[k] = x.y;
-  }
+    for (int k = 0; k < 8; ++k) {
+      const int p = DIN ? (g0.cb | g0.in_lane | roff(g0, 4 * k)) : tid + kTThreads * k;
+      const V x = s[bb + mem(p)];
+      vx[k] = x.x;
+      vy[k] = x.y;
+    }
+  };
+  load_next(base);
   for (;;) {
+    if (!DIN) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
-    lds_barrier();
+      for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
+      lds_barrier();
+    }
     const uint64_t tn = t + gridDim.x;
     const bool more = tn < a.ntiles;  // uniform
     // the last tile re-reads itself (unused): unconditional loads keep v in registers
     const uint64_t bn = more ? insert_zeros(tn, a.tbit, kTT) : base;
-#pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int p = tid + kTThreads * k;
-      const V x = s[bn + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]];
-      vx[k] = x.x;
-      vy[k] = x.y;
-    }
+    if (!DIN) load_next(bn);
     for (int b = 0; b < a.nblk; ++b) {
       const R* f = afr + b * kCFrag;
       R ur[16], ui[16];
@@ -273,25 +310,18 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
         ur[e] = f[e * 64 + lane];
         ui[e] = f[(16 + e) * 64 + lane];
       }
-      int q[kBQ], qs[kBQ];
-#pragma unroll
-      for (int j = 0; j < kBQ; ++j) {
-        q[j] = bq[b][j];
-        qs[j] = bq[b][kBQ + j];
-      }
-      auto roff = [&](int r) {
-        int o = 0;
-#pragma unroll
-        for (int j = 0; j < kBQ; ++j) o |= ((r >> j) & 1) << q[j];
-        return o;
-      };
-      const int in_lane = roff(grp), out_lane = roff(M::drow(lane, 0));
-      int cb = wave * kCols + col;  // this lane's column, then its tile position
-#pragma unroll
-      for (int j = 0; j < kBQ; ++j) cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
+      Geo g;
+      if (b == 0) g = g0;
+      else geo(b, g);
       V x[8];
+      if (DIN && b == 0) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(cb | in_lane | roff(4 * m))];
+        for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[m], vy[m]);
+        load_next(bn);  // the operands are taken; the next tile's loads go in flight
+      } else {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) x[m] = lds[tpad(g.cb | g.in_lane | roff(g, 4 * m))];
+      }
       // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
       // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2)
       typename M::acc_t p1[2], p2[2], p3[2];
@@ -311,22 +341,29 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
           p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
         }
       }
+      const bool to_hbm = DOUT && b == a.nblk - 1;
 #pragma unroll
       for (int m = 0; m < 8; ++m) {
         V o;
         const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
         o.x = a1 - a2;
         o.y = p3[m >> 2][m & 3] - a1 - a2;
-        lds[tpad(cb | out_lane | roff(16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0))))] = o;
+        const int p = g.cb | g.out_lane | roff(g, 16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0)));
+        if (to_hbm) s[base + mem(p)] = o;
+        else lds[tpad(p)] = o;
       }
       lds_barrier();
     }
-    // each thread stores, then (next iteration) refills, the same LDS positions, so
-    // no barrier is needed between the two
+    if (!DOUT) {
+      // without DIN each thread stores, then (next iteration) refills, the same LDS
+      // positions, so no barrier is needed between the two; with DIN the first
+      // block writes other positions, so a barrier follows the store
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const int p = tid + kTThreads * k;
-      s[base + uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]] = lds[tpad(p)];
+      for (int k = 0; k < kPer; ++k) {
+        const int p = tid + kTThreads * k;
+        s[base + mem(p)] = lds[tpad(p)];
+      }
+      if (DIN) lds_barrier();
     }
     if (!more) break;
     t = tn;
@@ -822,7 +859,7 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     YH_HIP(hipGetDevice(&dev));
     YH_HIP(hipGetDeviceProperties(&prop, dev));
     YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block5<R>, 256, 0));
-    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_tblocks<R>, kTThreads, 0));
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_tblocks<R, false, false>, kTThreads, 0));
     res[0] = prop.multiProcessorCount * std::max(1, per_cu);
     res[1] = prop.multiProcessorCount * std::max(1, per_cu_t);
   }
@@ -884,8 +921,32 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       }
     a.nblk = int(it.blocks.size());
     a.ntiles = uint64_t(1) << (nbits - kTT);
-    for (size_t bi = 0; bi < it.blocks.size(); ++bi) {
-      const BStep& st = steps[it.blocks[bi]];
+    // A block whose rows avoid memory bits 0..3 can take its operands from HBM
+    // (first) or put its results to HBM (last) with whole 16-amplitude runs; such a
+    // block is moved to the front / the end when it shares no bit with the blocks
+    // it moves past.
+    std::vector<size_t> order = it.blocks;
+    const uint64_t low = (uint64_t(1) << kTLow) - 1;
+    auto mask = [&](size_t k) { return step_mask(steps[order[k]]); };
+    bool din = false, dout = false;
+    for (size_t j = 0; j < order.size() && !din; ++j) {
+      bool ok = !(mask(j) & low);
+      for (size_t k = 0; k < j && ok; ++k) ok = !(mask(j) & mask(k));
+      if (ok) {
+        std::rotate(order.begin(), order.begin() + j, order.begin() + j + 1);
+        din = true;
+      }
+    }
+    for (size_t j = order.size(); j-- > (din ? 1u : 0u) && !dout;) {
+      bool ok = !(mask(j) & low);
+      for (size_t k = j + 1; k < order.size() && ok; ++k) ok = !(mask(j) & mask(k));
+      if (ok) {
+        std::rotate(order.begin() + j, order.begin() + j + 1, order.end());
+        dout = true;
+      }
+    }
+    for (size_t bi = 0; bi < order.size(); ++bi) {
+      const BStep& st = steps[order[bi]];
       int qs[kBQ];
       for (int j = 0; j < kBQ; ++j) {
         a.q[bi][j] = uint8_t(pos_of[st.p[j]]);
@@ -900,7 +961,18 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     YH_HIP(hipMemcpyAsync(dev + off, hst + off, sz * sizeof(R), hipMemcpyHostToDevice, w.stream));
     const uint64_t grid = std::min<uint64_t>(a.ntiles, uint64_t(res[1]));
     YH_TRY(w.time_begin(YH_K_TBLOCK));
-    hipLaunchKernelGGL(k_tblocks<R>, dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a, dev + off);
+    if (din && dout)
+      hipLaunchKernelGGL((k_tblocks<R, true, true>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
+                         dev + off);
+    else if (din)
+      hipLaunchKernelGGL((k_tblocks<R, true, false>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
+                         dev + off);
+    else if (dout)
+      hipLaunchKernelGGL((k_tblocks<R, false, true>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
+                         dev + off);
+    else
+      hipLaunchKernelGGL((k_tblocks<R, false, false>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
+                         dev + off);
     YH_HIP(hipGetLastError());
     YH_TRY(w.time_end());
     off += sz;
