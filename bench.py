@@ -30,12 +30,15 @@ HBM_PEAK_GBPS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 # HBM bytes per launch on the 28-qubit state, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 20 of k_tblocks,
-# 3 of k_block5), each scaled by the factor a plain copy of known size gives for
-# the same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
-# profiles/k_block5_brickwork28_pmc.md.
-TRAFFIC28_BYTES = {"tblock": int((2 * 2097560 + 4194304) * 1024),
-                   "fused": int((2 * 2097515 + 4194304) * 1024)}
+# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 20 of k_tblocks
+# in its four direct-in/out forms, 3 of k_block5), each scaled by the factor a plain
+# copy of known size gives for the same counter (2 for FETCH_SIZE, 1 for
+# WRITE_SIZE); summary in profiles/k_block5_brickwork28_pmc.md. Keyed by the
+# amplitude size in bytes (16 complex128, 8 complex64).
+TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2097678 + 4194304) * 1024),
+                        "fused": int((2 * 2097530 + 4194304) * 1024)},
+                   8: {"tblock": int((2 * 1048714 + 2097152) * 1024),
+                       "fused": int((2 * 1048780 + 2097152) * 1024)}}
 NBITS = 28
 DEPTH = 20
 SEED = 20
@@ -178,7 +181,7 @@ def main():
         """k_block5 (one block per launch) as HBM-bound, k_tblocks (several per
         launch) as MFMA-bound, from the launches' HIP-event averages."""
         flops_block = (1 << n) * 64 * 64 * 2 // 32
-        traffic = TRAFFIC28_BYTES if (n == 28 and amp_bytes == 16) else {}
+        traffic = TRAFFIC28_BYTES.get(amp_bytes, {}) if n == 28 else {}
         per = {}
         if "fused" in ks:
             r = roofline_of(ks, amp_bytes=amp_bytes, dom="fused")
