@@ -276,12 +276,13 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   uint64_t t = blockIdx.x;
   if (t >= a.ntiles) return;  // uniform
   uint64_t base = insert_zeros(t, a.tbit, kTT);
-  // DIN: the first block's operands of the next tile; else the next tile in thread order
-  R vx[8], vy[8];
   static_assert(kPer == 8, "eight amplitudes per thread");
   Geo g0;
   geo(0, g0);
-  auto load_next = [&](uint64_t bb) {
+  // the next tile's amplitudes: with DIN in block 0's MFMA operand layout, else in
+  // thread order (position tid + kTThreads k)
+  R vx[8], vy[8];
+  auto load_tile = [&](uint64_t bb) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int p = DIN ? (g0.cb | g0.in_lane | roff(g0, 4 * k)) : tid + kTThreads * k;
@@ -290,84 +291,115 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       vy[k] = x.y;
     }
   };
-  load_next(base);
-  for (;;) {
-    if (!DIN) {
+  auto fill = [&]() __attribute__((always_inline)) {
 #pragma unroll
-      for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
-      lds_barrier();
+    for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
+  };
+  // block b of the pass on the tile at tb; `first` (DIN) takes the operands from
+  // vx/vy and then loads the tile at bn into them, `to_hbm` (DOUT) writes the
+  // results to HBM instead of LDS. Both are constants where the block is inlined.
+  auto block = [&](int b, bool first, bool to_hbm, uint64_t tb, uint64_t bn) __attribute__((always_inline)) {
+    // the block's geometry first (its LDS reads return in order, so they go
+    // before the A values'), then the operands' reads, then the A values
+    Geo g;
+    if (b == 0) g = g0;
+    else geo(b, g);
+    V x[8];
+    if (DIN && first) {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[m], vy[m]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(g.cb | g.in_lane | roff(g, 4 * m))];
     }
-    const uint64_t tn = t + gridDim.x;
-    const bool more = tn < a.ntiles;  // uniform
-    // the last tile re-reads itself (unused): unconditional loads keep v in registers
-    const uint64_t bn = more ? insert_zeros(tn, a.tbit, kTT) : base;
-    if (!DIN) load_next(bn);
-    for (int b = 0; b < a.nblk; ++b) {
-      const R* f = afr + b * kCFrag;
-      R ur[16], ui[16];
+    const R* f = afr + b * kCFrag;
+    R ur[16], ui[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        ur[e] = f[e * 64 + lane];
-        ui[e] = f[(16 + e) * 64 + lane];
-      }
-      Geo g;
-      if (b == 0) g = g0;
-      else geo(b, g);
-      V x[8];
-      if (DIN && b == 0) {
+    for (int e = 0; e < 16; ++e) {
+      ur[e] = f[e * 64 + lane];
+      ui[e] = f[(16 + e) * 64 + lane];
+    }
+    // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
+    // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2)
+    typename M::acc_t p1[2], p2[2], p3[2];
 #pragma unroll
-        for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[m], vy[m]);
-        load_next(bn);  // the operands are taken; the next tile's loads go in flight
-      } else {
+    for (int rt = 0; rt < 2; ++rt) {
+      p1[rt] = typename M::acc_t{0, 0, 0, 0};
+      p2[rt] = typename M::acc_t{0, 0, 0, 0};
+      p3[rt] = typename M::acc_t{0, 0, 0, 0};
+    }
 #pragma unroll
-        for (int m = 0; m < 8; ++m) x[m] = lds[tpad(g.cb | g.in_lane | roff(g, 4 * m))];
-      }
-      // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
-      // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2)
-      typename M::acc_t p1[2], p2[2], p3[2];
+    for (int ks = 0; ks < 8; ++ks) {
+      const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
 #pragma unroll
       for (int rt = 0; rt < 2; ++rt) {
-        p1[rt] = typename M::acc_t{0, 0, 0, 0};
-        p2[rt] = typename M::acc_t{0, 0, 0, 0};
-        p3[rt] = typename M::acc_t{0, 0, 0, 0};
+        p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
+        p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
+        p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
       }
-#pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
-          p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
-          p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
-          p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
-        }
-      }
-      const bool to_hbm = DOUT && b == a.nblk - 1;
-#pragma unroll
-      for (int m = 0; m < 8; ++m) {
-        V o;
-        const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
-        o.x = a1 - a2;
-        o.y = p3[m >> 2][m & 3] - a1 - a2;
-        const int p = g.cb | g.out_lane | roff(g, 16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0)));
-        if (to_hbm) s[base + mem(p)] = o;
-        else lds[tpad(p)] = o;
-      }
-      lds_barrier();
     }
+    if (DIN && first) {
+      // the operands are consumed: the next tile's loads reuse their registers
+      // (kept below the MFMAs, so no second register set is needed)
+      __builtin_amdgcn_sched_barrier(0);
+      load_tile(bn);
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      V o;
+      const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
+      o.x = a1 - a2;
+      o.y = p3[m >> 2][m & 3] - a1 - a2;
+      const int p = g.cb | g.out_lane | roff(g, 16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0)));
+      if (to_hbm) s[tb + mem(p)] = o;
+      else lds[tpad(p)] = o;
+    }
+    lds_barrier();
+  };
+  // One tile: the next tile's loads are issued before this tile's stores, and every
+  // wait on them follows those stores in straight-line code (the fill, or the next
+  // tile's first block after the peeled first tile), so the stores stay in flight
+  // while the next tile starts. A pass holds at least two blocks (the host runs a
+  // block alone with k_block5), so the first and the DOUT block differ.
+  auto run_tile = [&](uint64_t tb, uint64_t bn) __attribute__((always_inline)) {
+    if (!DIN) load_tile(bn);
+    const int nmid = DOUT ? a.nblk - 1 : a.nblk;
+    block(0, true, false, tb, bn);
+    for (int b = 1; b < nmid; ++b) block(b, false, false, tb, bn);
+    if (DOUT) block(a.nblk - 1, false, true, tb, bn);
     if (!DOUT) {
-      // without DIN each thread stores, then (next iteration) refills, the same LDS
-      // positions, so no barrier is needed between the two; with DIN the first
-      // block writes other positions, so a barrier follows the store
+      // without DIN each thread stores, then refills, the same LDS positions, so no
+      // barrier is needed between the two; with DIN the next tile's first block
+      // writes other positions, so a barrier follows the store
 #pragma unroll
       for (int k = 0; k < kPer; ++k) {
         const int p = tid + kTThreads * k;
-        s[base + mem(p)] = lds[tpad(p)];
+        s[tb + mem(p)] = lds[tpad(p)];
       }
       if (DIN) lds_barrier();
     }
-    if (!more) break;
+  };
+  // the last tile of a workgroup loads itself again (unused): unconditional loads
+  // keep the operands in registers
+  auto next_base = [&](uint64_t tn) { return tn < a.ntiles ? insert_zeros(tn, a.tbit, kTT) : base; };
+  load_tile(base);
+  if (!DIN) {
+    fill();
+    lds_barrier();
+  }
+  uint64_t tn = t + gridDim.x;
+  uint64_t bn = next_base(tn);
+  run_tile(base, bn);  // peeled: the loop is entered with the same operations in flight as at its back edge
+  while (tn < a.ntiles) {  // uniform
+    if (!DIN) {
+      fill();
+      lds_barrier();
+    }
     t = tn;
     base = bn;
+    tn = t + gridDim.x;
+    bn = next_base(tn);
+    run_tile(base, bn);
   }
 }
 
@@ -920,6 +952,7 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
         pos_of[b] = k++;
       }
     a.nblk = int(it.blocks.size());
+    if (a.nblk < 2 || a.nblk > kPassMax) return fail(YH_ERR_UNSUPPORTED, "block pass size out of range");
     a.ntiles = uint64_t(1) << (nbits - kTT);
     // A block whose rows avoid memory bits 0..3 can take its operands from HBM
     // (first) or put its results to HBM (last) with whole 16-amplitude runs; such a
