@@ -455,6 +455,7 @@ struct BStep {
   bool block = false;
   size_t gate = 0;               // single launch: index into the gate list
   int p[kBQ] = {0, 0, 0, 0, 0};  // block: row bit j <-> memory bit p[j]
+  uint64_t support = 0;          // block: bits its gates act on (p[] adds identity bits)
   std::vector<uint32_t> members; // block: gate indices, circuit order
 };
 
@@ -611,7 +612,10 @@ struct Planner {
         for (int b = 0; b < nbits && np < kBQ; ++b)
           if (!((bestQ >> b) & 1)) s.p[np++] = b;
         s.members = best;
-        for (uint32_t i : best) done[i] = 1;
+        for (uint32_t i : best) {
+          s.support |= bg[i].mask;
+          done[i] = 1;
+        }
         steps.push_back(std::move(s));
       }
       size_t k = 0;
@@ -831,7 +835,26 @@ uint64_t step_mask(const BStep& st) {
   return m;
 }
 
-std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
+// A block acting on fewer than 5 bits is an identity on the rest of its rows, so in
+// a multi-block pass those rows may be any other bits of the tile: bits above the
+// low run are taken first, which lets a block that acts on none of memory bits
+// 0..3 read or write HBM directly.
+void pad_in_tile(BStep& st, uint64_t tile) {
+  const uint64_t sup = st.support;
+  int np = 0;
+  for (int b = 0; b < 64 && np < kBQ; ++b)
+    if ((sup >> b) & 1) st.p[np++] = b;
+  for (int b = 63; b >= kTLow && np < kBQ; --b)
+    if (((tile & ~sup) >> b) & 1) st.p[np++] = b;
+  for (int b = 0; b < kTLow && np < kBQ; ++b)
+    if (((tile & ~sup) >> b) & 1) st.p[np++] = b;
+}
+
+// Consecutive blocks are packed greedily into passes, each block by its 5 rows
+// (`by_support` false) or by the bits it acts on (true: its identity rows go to
+// other tile bits, and blocks that meet only on identity rows commute).
+std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits, bool by_support) {
+  auto bits_of = [&](const BStep& st) { return by_support ? st.support : step_mask(st); };
   std::vector<PassItem> items;
   size_t i = 0;
   while (i < steps.size()) {
@@ -851,7 +874,7 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
       uint64_t cur = (uint64_t(1) << kTLow) - 1, blocked = 0;
       std::vector<size_t> left;
       for (size_t k : rem) {
-        const uint64_t bm = step_mask(steps[k]);
+        const uint64_t bm = bits_of(steps[k]);
         if (nbits > kTT && !(bm & blocked) && popc(cur | bm) <= kTT && it.blocks.size() < size_t(kPassMax)) {
           cur |= bm;
           it.blocks.push_back(k);
@@ -862,7 +885,7 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits) {
       }
       if (it.blocks.empty()) {  // n <= 12: one block per pass (k_block5)
         it.blocks.push_back(left.front());
-        cur |= step_mask(steps[left.front()]);
+        cur |= bits_of(steps[left.front()]);
         left.erase(left.begin());
       }
       it.bits = cur;
@@ -895,7 +918,19 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     res[0] = prop.multiProcessorCount * std::max(1, per_cu);
     res[1] = prop.multiProcessorCount * std::max(1, per_cu_t);
   }
-  const std::vector<PassItem> items = group_passes(steps, nbits);
+  // of the two packings, the one with the lower modelled time: a pass of k >= 2
+  // blocks costs about 0.46 + k block-times, a block alone in its pass 1.8
+  auto cost = [](const std::vector<PassItem>& v) {
+    double c = 0;
+    for (const PassItem& it : v)
+      if (!it.single) c += it.blocks.size() == 1 ? 1.8 : 0.46 + double(it.blocks.size());
+    return c;
+  };
+  std::vector<PassItem> items = group_passes(steps, nbits, false);
+  {
+    std::vector<PassItem> alt = group_passes(steps, nbits, true);
+    if (cost(alt) < cost(items)) items.swap(alt);
+  }
   // Block matrices are built on the host while the GPU runs the previous passes:
   // each pass's operands go to their own part of a pinned staging area and are
   // copied with an async copy ordered before its launch (nothing else writes the
@@ -954,16 +989,23 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     a.nblk = int(it.blocks.size());
     if (a.nblk < 2 || a.nblk > kPassMax) return fail(YH_ERR_UNSUPPORTED, "block pass size out of range");
     a.ntiles = uint64_t(1) << (nbits - kTT);
+    std::vector<BStep> ps;  // the pass's blocks with their identity rows in the tile
+    for (size_t k : it.blocks) {
+      ps.push_back(steps[k]);
+      pad_in_tile(ps.back(), tb);
+    }
     // A block whose rows avoid memory bits 0..3 can take its operands from HBM
     // (first) or put its results to HBM (last) with whole 16-amplitude runs; such a
-    // block is moved to the front / the end when it shares no bit with the blocks
-    // it moves past.
-    std::vector<size_t> order = it.blocks;
+    // block is moved to the front / the end when it acts on no bit of the blocks it
+    // moves past.
+    std::vector<size_t> order(ps.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
     const uint64_t low = (uint64_t(1) << kTLow) - 1;
-    auto mask = [&](size_t k) { return step_mask(steps[order[k]]); };
+    auto rows = [&](size_t k) { return step_mask(ps[order[k]]); };
+    auto mask = [&](size_t k) { return ps[order[k]].support; };
     bool din = false, dout = false;
     for (size_t j = 0; j < order.size() && !din; ++j) {
-      bool ok = !(mask(j) & low);
+      bool ok = !(rows(j) & low);
       for (size_t k = 0; k < j && ok; ++k) ok = !(mask(j) & mask(k));
       if (ok) {
         std::rotate(order.begin(), order.begin() + j, order.begin() + j + 1);
@@ -971,7 +1013,7 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       }
     }
     for (size_t j = order.size(); j-- > (din ? 1u : 0u) && !dout;) {
-      bool ok = !(mask(j) & low);
+      bool ok = !(rows(j) & low);
       for (size_t k = j + 1; k < order.size() && ok; ++k) ok = !(mask(j) & mask(k));
       if (ok) {
         std::rotate(order.begin() + j, order.begin() + j + 1, order.end());
@@ -979,7 +1021,7 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       }
     }
     for (size_t bi = 0; bi < order.size(); ++bi) {
-      const BStep& st = steps[order[bi]];
+      const BStep& st = ps[order[bi]];
       int qs[kBQ];
       for (int j = 0; j < kBQ; ++j) {
         a.q[bi][j] = uint8_t(pos_of[st.p[j]]);

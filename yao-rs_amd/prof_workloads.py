@@ -12,6 +12,7 @@ usage: python yao-rs_amd/prof_workloads.py WORKLOAD [nbits] [depth]
   k1q_bits   Rx on every qubit of an nbits complex128 state (4 gates per qubit),
              printing k_1q's average launch time per target bit (HIP events)
   dm         noisy brickwork on a density matrix (fused gate+channel passes)
+  dmblk      the same on the superoperator block path
 Each workload prints one line naming itself before it touches the GPU.
 """
 import os
@@ -71,10 +72,10 @@ def main():
             us = 1e3 * ms.value / c.value
             print(f"[prof] k_1q target bit {n - 1 - loc:2d}: {us:8.1f} us, "
                   f"{2 * 16 * (1 << n) / (us * 1e-6) / 1e12:.2f} TB/s", flush=True)
-    elif w == "dm":
+    elif w in ("dm", "dmblk"):
         dm = yaohip.DensityMatrix.zero_state(n)
-        dm.apply(Circuit.qubits(n, noisy_brickwork(n, depth, 3)), fused=True)
-        print(f"[prof] dm trace {dm.trace().real:.15f}", flush=True)
+        dm.apply(Circuit.qubits(n, noisy_brickwork(n, depth, 3)), fused=True if w == "dm" else "blocks")
+        print(f"[prof] {w} trace {dm.trace().real:.15f}", flush=True)
     else:
         raise SystemExit(f"unknown workload {w}")
 
