@@ -35,10 +35,10 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # copy of known size gives for the same counter (2 for FETCH_SIZE, 1 for
 # WRITE_SIZE); summary in profiles/k_block5_brickwork28_pmc.md. Keyed by the
 # amplitude size in bytes (16 complex128, 8 complex64).
-TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2097678 + 4194304) * 1024),
-                        "fused": int((2 * 2097530 + 4194304) * 1024)},
-                   8: {"tblock": int((2 * 1048714 + 2097152) * 1024),
-                       "fused": int((2 * 1048780 + 2097152) * 1024)}}
+TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2097696 + 4194304) * 1024),
+                        "fused": int((2 * 2097495 + 4194304) * 1024)},
+                   8: {"tblock": int((2 * 1048740 + 2097152) * 1024),
+                       "fused": int((2 * 1048776 + 2097152) * 1024)}}
 NBITS = 28
 DEPTH = 20
 SEED = 20
