@@ -210,11 +210,18 @@ constexpr int kTThreads = 512;         // 8 waves: one 16-column step each per b
 constexpr int kPassMax = 5;            // blocks per pass
 constexpr int kCFrag = 32 * 64;        // compact A values of one block
 
+// A block of the pass is a 32 x 32 matrix on 5 tile bits, or (bit b of `small`) a
+// 16 x 16 matrix on 4: then row bit 4 does not exist and its q / qs entries hold
+// kNoBit, a position above every tile bit (inserting a zero there, or setting it
+// for row bit 4 of a row index below 16, changes nothing).
+constexpr int kNoBit = 30;
+
 struct TBlockArgs {
   uint8_t tbit[kTT];          // memory bit of tile position k, ascending (0..3 first)
   uint8_t q[kPassMax][kBQ];   // tile position of row bit j of block b
   uint8_t qs[kPassMax][kBQ];  // the same positions, ascending
   int32_t nblk;
+  uint32_t small;             // bit b: block b is a 4-bit (16 x 16) block
   uint64_t ntiles;            // 2^(n - 12)
 };
 
@@ -249,9 +256,12 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   // memory offset (from the tile's base) of tile position p
   auto mem = [&](int p) { return uint64_t(p & ((1 << kTLow) - 1)) + hoff[p >> kTLow]; };
   // geometry of block b for this lane: row offsets and the column's tile position
+  // a 5-bit block views the tile as 32 rows x 128 columns, one 16-column group per
+  // wave (column cb); a 4-bit block as 16 x 256, two groups per wave (cb, cb1)
   struct Geo {
     int q[kBQ];
-    int cb, in_lane, out_lane;
+    int cb, cb1, in_lane, out_lane;
+    bool small;
   };
   auto roff = [](const Geo& g, int r) {
     int o = 0;
@@ -266,12 +276,27 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       g.q[j] = bq[b][j];
       qs[j] = bq[b][kBQ + j];
     }
+    g.small = (a.small >> b) & 1;
     g.in_lane = roff(g, grp);
     g.out_lane = roff(g, M::drow(lane, 0));
-    int cb = wave * kCols + col;  // this lane's column, then its tile position
+    // this lane's column(s), then their tile positions
+    int cb = g.small ? wave * 2 * kCols + col : wave * kCols + col, cb1 = cb + kCols;
 #pragma unroll
-    for (int j = 0; j < kBQ; ++j) cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
+    for (int j = 0; j < kBQ; ++j) {
+      cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
+      cb1 = ((cb1 >> qs[j]) << (qs[j] + 1)) | (cb1 & ((1 << qs[j]) - 1));
+    }
     g.cb = cb;
+    g.cb1 = cb1;
+  };
+  // tile position of operand m (B fragment: k-step m, or group m / 4 and k-step
+  // m % 4) and of result m (row tile m / 4, or group m / 4; element m % 4)
+  auto xpos = [&](const Geo& g, int m) {
+    return g.small ? ((m >> 2 ? g.cb1 : g.cb) | g.in_lane | roff(g, 4 * (m & 3))) : (g.cb | g.in_lane | roff(g, 4 * m));
+  };
+  auto opos = [&](const Geo& g, int m) {
+    const int d = M::drow(lane, m & 3) - M::drow(lane, 0);
+    return g.small ? ((m >> 2 ? g.cb1 : g.cb) | g.out_lane | roff(g, d)) : (g.cb | g.out_lane | roff(g, 16 * (m >> 2) + d));
   };
   uint64_t t = blockIdx.x;
   if (t >= a.ntiles) return;  // uniform
@@ -285,7 +310,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   auto load_tile = [&](uint64_t bb) __attribute__((always_inline)) {
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-      const int p = DIN ? (g0.cb | g0.in_lane | roff(g0, 4 * k)) : tid + kTThreads * k;
+      const int p = DIN ? xpos(g0, k) : tid + kTThreads * k;
       const V x = s[bb + mem(p)];
       vx[k] = x.x;
       vy[k] = x.y;
@@ -310,17 +335,12 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[m], vy[m]);
     } else {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(g.cb | g.in_lane | roff(g, 4 * m))];
+      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(xpos(g, m))];
     }
     const R* f = afr + b * kCFrag;
-    R ur[16], ui[16];
-#pragma unroll
-    for (int e = 0; e < 16; ++e) {
-      ur[e] = f[e * 64 + lane];
-      ui[e] = f[(16 + e) * 64 + lane];
-    }
     // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
-    // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2)
+    // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2). Accumulator
+    // pair index: row tile (5-bit block) or column group (4-bit block).
     typename M::acc_t p1[2], p2[2], p3[2];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
@@ -328,14 +348,38 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       p2[rt] = typename M::acc_t{0, 0, 0, 0};
       p3[rt] = typename M::acc_t{0, 0, 0, 0};
     }
+    if (g.small) {
+      R ur[4], ui[4];
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-      const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+      for (int e = 0; e < 4; ++e) {
+        ur[e] = f[e * 64 + lane];
+        ui[e] = f[(4 + e) * 64 + lane];
+      }
 #pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
-        p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
-        p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+      for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+        for (int gi = 0; gi < 2; ++gi) {
+          const V xv = x[gi * 4 + ks];
+          p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
+          p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
+          p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
+        }
+    } else {
+      R ur[16], ui[16];
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        ur[e] = f[e * 64 + lane];
+        ui[e] = f[(16 + e) * 64 + lane];
+      }
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
+          p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
+          p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+        }
       }
     }
     if (DIN && first) {
@@ -350,7 +394,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
       o.x = a1 - a2;
       o.y = p3[m >> 2][m & 3] - a1 - a2;
-      const int p = g.cb | g.out_lane | roff(g, 16 * (m >> 2) + (M::drow(lane, m & 3) - M::drow(lane, 0)));
+      const int p = opos(g, m);
       if (to_hbm) s[tb + mem(p)] = o;
       else lds[tpad(p)] = o;
     }
@@ -456,6 +500,7 @@ struct BStep {
   size_t gate = 0;               // single launch: index into the gate list
   int p[kBQ] = {0, 0, 0, 0, 0};  // block: row bit j <-> memory bit p[j]
   uint64_t support = 0;          // block: bits its gates act on (p[] adds identity bits)
+  bool small = false;            // block: applied as a 4-bit block on p[0..3] (p[4] identity)
   std::vector<uint32_t> members; // block: gate indices, circuit order
 };
 
@@ -817,6 +862,19 @@ void to_compact(const std::vector<cd>& U, R* o) {
     }
 }
 
+// the same for a 4-bit block, from the 32 x 32 matrix built with row bit 4 an
+// identity bit (so the upper-left 16 x 16 corner is the block): value e of lane l
+// is Ur (e < 4) or Ui (e >= 4) at row l % 16, column 4 (e % 4) + l / 16
+template <class R>
+void to_compact_small(const std::vector<cd>& U, R* o) {
+  for (int e = 0; e < 8; ++e)
+    for (int l = 0; l < 64; ++l) {
+      const int r = l % 16, c = 4 * (e % 4) + l / 16;
+      const cd u = U[size_t(r) * kBD + c];
+      o[e * 64 + l] = R(e < 4 ? u.re : u.im);
+    }
+}
+
 // Launch order of a block plan: gates run alone stay where they are; each run of
 // consecutive blocks is split into passes of k_tblocks. A pass starts at the first
 // block not yet placed and takes, in order, every later block whose bits keep the
@@ -831,16 +889,19 @@ struct PassItem {
 
 uint64_t step_mask(const BStep& st) {
   uint64_t m = 0;
-  for (int j = 0; j < kBQ; ++j) m |= uint64_t(1) << st.p[j];
+  for (int j = 0; j < (st.small ? kBQ - 1 : kBQ); ++j) m |= uint64_t(1) << st.p[j];
   return m;
 }
 
 // A block acting on fewer than 5 bits is an identity on the rest of its rows, so in
 // a multi-block pass those rows may be any other bits of the tile: bits above the
 // low run are taken first, which lets a block that acts on none of memory bits
-// 0..3 read or write HBM directly.
+// 0..3 read or write HBM directly. A block acting on at most 4 bits runs as a
+// 16 x 16 block on p[0..3] (half the MFMA work); p[4] is then one more identity
+// bit, used only to build its matrix.
 void pad_in_tile(BStep& st, uint64_t tile) {
   const uint64_t sup = st.support;
+  st.small = popc(sup) <= kBQ - 1;
   int np = 0;
   for (int b = 0; b < 64 && np < kBQ; ++b)
     if ((sup >> b) & 1) st.p[np++] = b;
@@ -1022,15 +1083,21 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     }
     for (size_t bi = 0; bi < order.size(); ++bi) {
       const BStep& st = ps[order[bi]];
+      const int nq = st.small ? kBQ - 1 : kBQ;
       int qs[kBQ];
       for (int j = 0; j < kBQ; ++j) {
-        a.q[bi][j] = uint8_t(pos_of[st.p[j]]);
-        qs[j] = pos_of[st.p[j]];
+        qs[j] = j < nq ? pos_of[st.p[j]] : kNoBit;
+        a.q[bi][j] = uint8_t(qs[j]);
       }
       std::sort(qs, qs + kBQ);
       for (int j = 0; j < kBQ; ++j) a.qs[bi][j] = uint8_t(qs[j]);
       block_matrix(st, bg, U);
-      to_compact<R>(U, hst + off + bi * kCFrag);
+      if (st.small) {
+        a.small |= 1u << bi;
+        to_compact_small<R>(U, hst + off + bi * kCFrag);
+      } else {
+        to_compact<R>(U, hst + off + bi * kCFrag);
+      }
     }
     const size_t sz = it.blocks.size() * kCFrag;
     YH_HIP(hipMemcpyAsync(dev + off, hst + off, sz * sizeof(R), hipMemcpyHostToDevice, w.stream));
