@@ -171,6 +171,15 @@ int yh_block_plan_stats(int nbits, const yh_op *ops, size_t nops, int64_t out[3]
  * its own launch. Fills at most `cap` steps; *nsteps receives the plan length.
  * No reference counterpart (planning introspection). */
 int yh_block_plan(int nbits, const yh_op *ops, size_t nops, int32_t *bits, size_t cap, size_t *nsteps);
+/* Host-only launch plan of the block path, pass by pass: dm = 0 for yh_apply_fused
+ * (max_fuse_qubits 5) on a register of nbits qubits, dm = 1 for yh_dm_apply_blocks
+ * on a density matrix of nbits qubits. out[0] block-kernel launches (one HBM pass
+ * each), out[1] multi-block passes, out[2] blocks in them, out[3] of those applied
+ * as 16 x 16 blocks (acting on at most 4 bits), out[4] multi-block passes whose
+ * first block reads HBM directly, out[5] whose last block writes HBM directly,
+ * out[6] elements run as their own launch. No device call; no reference
+ * counterpart (planning introspection). */
+int yh_block_pass_stats(int nbits, const yh_op *ops, size_t nops, int dm, int64_t out[7]);
 /* Host-only plan of yh_apply_fused for a circuit on nbits qubits: out[0] HBM passes
  * of the tile kernel, out[1] gate groups, out[2] gates placed in tiles, out[3]
  * gates run as their own launch. No device call. */

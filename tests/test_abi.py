@@ -191,3 +191,53 @@ def test_circuit_num_params_matches_python_mirror():
     n = ctypes.c_size_t()
     A.check(A.lib().yh_circuit_num_params(c.ops.arr, c.ops.n, ctypes.byref(n)))
     assert n.value == c.num_params() == len(c.parameters())
+
+
+def _pass_stats(n, els, dm):
+    c = Circuit.qubits(n, els)
+    out = (ctypes.c_int64 * 7)()
+    A.check(A.lib().yh_block_pass_stats(n, c.ops.arr, c.ops.n, dm, out))
+    return list(out)
+
+
+def test_block_pass_stats_brickwork28():
+    """The 28-qubit depth-20 brickwork's launch plan: 48 blocks in 23 HBM passes,
+    20 of them multi-block; every block is in exactly one pass; most multi-block
+    passes read (first block) and write (last block) HBM directly."""
+    from fixture_circuits import brickwork
+    els = brickwork(28, 20, 20)
+    launches, multi, in_multi, small, din, dout, alone = _pass_stats(28, els, 0)
+    plan = (ctypes.c_int64 * 3)()
+    c = Circuit.qubits(28, els)
+    A.check(A.lib().yh_block_plan_stats(28, c.ops.arr, c.ops.n, plan))
+    assert plan[0] == 48 and alone == 0
+    assert (launches - multi) + in_multi == plan[0]  # blocks alone + blocks in passes
+    assert launches <= 23 and multi >= 20
+    assert din >= 16 and dout >= 12 and small <= in_multi
+
+
+def test_block_pass_stats_density_matrix():
+    """Noisy brickwork on a density matrix: every superoperator block acts on the
+    row and column bits of at most two qubits, so every block in a multi-block
+    pass is a 16 x 16 block, and its identity row lands above memory bits 0..3,
+    so every pass reads HBM directly."""
+    from fixture_circuits import noisy_brickwork
+    launches, multi, in_multi, small, din, dout, alone = _pass_stats(14, noisy_brickwork(14, 10, 3), 1)
+    assert alone == 0 and launches == multi and multi <= 20
+    assert small == in_multi == 65
+    assert din == multi and dout >= multi // 2
+
+
+def test_block_pass_stats_invariants():
+    """Random 16-qubit circuits: counts are consistent with the block plan."""
+    from fixture_circuits import random_circuit
+    for seed in range(3):
+        els = random_circuit(16, 300, seed)
+        launches, multi, in_multi, small, din, dout, alone = _pass_stats(16, els, 0)
+        plan = (ctypes.c_int64 * 3)()
+        c = Circuit.qubits(16, els)
+        A.check(A.lib().yh_block_plan_stats(16, c.ops.arr, c.ops.n, plan))
+        assert (launches - multi) + in_multi == plan[0]
+        assert alone == plan[2]
+        assert 0 < small < in_multi and din <= multi and dout <= multi
+        assert in_multi <= 5 * multi

@@ -2,6 +2,7 @@
 vectors. Bars: amplitudes within 1e-12 relative (max-abs error / max-abs amplitude)
 in complex128; probabilities of the full distribution and sampled outcomes
 bit-exact; complex64 within 2e-5 relative (its error is printed)."""
+import ctypes
 import math
 
 import numpy as np
@@ -11,6 +12,7 @@ from conftest import load_golden, rel_err, seeded_state
 from fixture_circuits import (NAMED, brickwork, elements_from_case, parse_matrix, parse_state, qft_with_swaps,
                               random_circuit)
 from oracle import oracle as O
+from yaohip import _abi as A
 from yaohip.circuit import (Circuit, Gate, OperatorPolynomial, OperatorString, Op, PositionedGate, control, lower,
                             put, tfim_hamiltonian)
 
@@ -333,6 +335,24 @@ def test_block5_qft_and_controlled_x(gpu):
     print("[case] block5 qft16 + controlled X", flush=True)
     got = run_gpu(gpu, n, psi, els, fused=5)
     assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_block_passes_with_16x16_blocks_vs_oracle(gpu, dtype):
+    """A random 16-qubit circuit whose launch plan puts blocks acting on at most 4
+    bits into multi-block passes (applied as 16 x 16 MFMA blocks): checked against
+    the oracle in both precisions."""
+    n = 16
+    els = random_circuit(n, 300, 0)
+    c = Circuit.qubits(n, els)
+    st = (ctypes.c_int64 * 7)()
+    A.check(A.lib().yh_block_pass_stats(n, c.ops.arr, c.ops.n, 0, st))
+    assert st[3] > 0 and st[1] > 0  # 16 x 16 blocks in multi-block passes
+    psi = seeded_state(n, 71)
+    print(f"[case] block passes with 16x16 blocks {dtype}", flush=True)
+    got = run_gpu(gpu, n, psi, els, dtype, fused=5)
+    err = rel_err(got, O.apply(n, psi, lower(els)))
+    assert err < (2e-5 if dtype == "c64" else TOL)
 
 
 def test_block5_complex64_error(gpu):

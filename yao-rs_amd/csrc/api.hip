@@ -618,6 +618,13 @@ int yh_block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]
   return block_plan_stats(nbits, ops, nops, out);
 }
 
+int yh_block_pass_stats(int nbits, const yh_op* ops, size_t nops, int dm, int64_t out[7]) {
+  YH_GUARD(out, YH_ERR_INVALID_ARG, "null argument");
+  YH_GUARD(dm == 0 || dm == 1, YH_ERR_INVALID_ARG, "dm must be 0 or 1");
+  YH_TRY(yh_validate(nbits, ops, nops));
+  return block_pass_stats(nbits, ops, nops, dm, out);
+}
+
 int yh_instruct(yh_reg* reg, const yh_op* op) {
   YH_GUARD(reg && op, YH_ERR_INVALID_ARG, "null argument");
   YH_GUARD(op->elem == YH_ELEM_GATE, YH_ERR_INVALID_ARG, "yh_instruct takes a gate element");

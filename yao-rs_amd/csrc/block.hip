@@ -958,6 +958,84 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits, b
   return items;
 }
 
+// One launch of a block plan, host side: a gate on its own path, a block alone in
+// its HBM pass (k_block5), or a multi-block pass (k_tblocks) with its blocks in
+// application order, their identity rows placed in the tile, and the direct-in /
+// direct-out forms it allows.
+struct PassPlan {
+  int kind = 0;               // 0 gate alone, 1 block alone, 2 multi-block pass
+  size_t step = 0;            // kinds 0 and 1: index into the plan
+  std::vector<BStep> blocks;  // kind 2
+  uint64_t tile = 0;          // kind 2: the 12 tile bits
+  bool din = false, dout = false;
+};
+
+int plan_passes(const std::vector<BStep>& steps, int nbits, std::vector<PassPlan>& out) {
+  // of the two packings, the one with the lower modelled time: a pass of k >= 2
+  // blocks costs about 0.46 + k block-times, a block alone in its pass 1.8
+  auto cost = [](const std::vector<PassItem>& v) {
+    double c = 0;
+    for (const PassItem& it : v)
+      if (!it.single) c += it.blocks.size() == 1 ? 1.8 : 0.46 + double(it.blocks.size());
+    return c;
+  };
+  std::vector<PassItem> items = group_passes(steps, nbits, false);
+  {
+    std::vector<PassItem> alt = group_passes(steps, nbits, true);
+    if (cost(alt) < cost(items)) items.swap(alt);
+  }
+  out.clear();
+  for (const PassItem& it : items) {
+    PassPlan pp;
+    if (it.single || it.blocks.size() == 1) {
+      pp.kind = it.single ? 0 : 1;
+      pp.step = it.single ? it.step : it.blocks[0];
+      out.push_back(std::move(pp));
+      continue;
+    }
+    // tile bits = the pass's bits padded with the lowest free bits to exactly 12
+    uint64_t tb = it.bits;
+    for (int b = 0; popc(tb) < kTT && b < nbits; ++b) tb |= uint64_t(1) << b;
+    if (popc(tb) != kTT || nbits <= kTT) return fail(YH_ERR_UNSUPPORTED, "block pass tile does not fit");
+    if (it.blocks.size() > size_t(kPassMax)) return fail(YH_ERR_UNSUPPORTED, "block pass size out of range");
+    pp.kind = 2;
+    pp.tile = tb;
+    std::vector<BStep> ps;  // the pass's blocks with their identity rows in the tile
+    for (size_t k : it.blocks) {
+      ps.push_back(steps[k]);
+      pad_in_tile(ps.back(), tb);
+    }
+    // A block whose rows avoid memory bits 0..3 can take its operands from HBM
+    // (first) or put its results to HBM (last) with whole 16-amplitude runs; such a
+    // block is moved to the front / the end when it acts on no bit of the blocks it
+    // moves past.
+    std::vector<size_t> order(ps.size());
+    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+    const uint64_t low = (uint64_t(1) << kTLow) - 1;
+    auto rows = [&](size_t k) { return step_mask(ps[order[k]]); };
+    auto mask = [&](size_t k) { return ps[order[k]].support; };
+    for (size_t j = 0; j < order.size() && !pp.din; ++j) {
+      bool ok = !(rows(j) & low);
+      for (size_t k = 0; k < j && ok; ++k) ok = !(mask(j) & mask(k));
+      if (ok) {
+        std::rotate(order.begin(), order.begin() + j, order.begin() + j + 1);
+        pp.din = true;
+      }
+    }
+    for (size_t j = order.size(); j-- > (pp.din ? 1u : 0u) && !pp.dout;) {
+      bool ok = !(rows(j) & low);
+      for (size_t k = j + 1; k < order.size() && ok; ++k) ok = !(mask(j) & mask(k));
+      if (ok) {
+        std::rotate(order.begin() + j, order.begin() + j + 1, order.end());
+        pp.dout = true;
+      }
+    }
+    for (size_t k : order) pp.blocks.push_back(std::move(ps[k]));
+    out.push_back(std::move(pp));
+  }
+  return YH_OK;
+}
+
 }  // namespace
 
 // Launches a block plan on a state of `nbits` bits; steps that are not blocks go
@@ -979,26 +1057,14 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     res[0] = prop.multiProcessorCount * std::max(1, per_cu);
     res[1] = prop.multiProcessorCount * std::max(1, per_cu_t);
   }
-  // of the two packings, the one with the lower modelled time: a pass of k >= 2
-  // blocks costs about 0.46 + k block-times, a block alone in its pass 1.8
-  auto cost = [](const std::vector<PassItem>& v) {
-    double c = 0;
-    for (const PassItem& it : v)
-      if (!it.single) c += it.blocks.size() == 1 ? 1.8 : 0.46 + double(it.blocks.size());
-    return c;
-  };
-  std::vector<PassItem> items = group_passes(steps, nbits, false);
-  {
-    std::vector<PassItem> alt = group_passes(steps, nbits, true);
-    if (cost(alt) < cost(items)) items.swap(alt);
-  }
+  std::vector<PassPlan> plan;
+  YH_TRY(plan_passes(steps, nbits, plan));
   // Block matrices are built on the host while the GPU runs the previous passes:
   // each pass's operands go to their own part of a pinned staging area and are
   // copied with an async copy ordered before its launch (nothing else writes the
   // area in this call).
   size_t total = 0;
-  for (const PassItem& it : items)
-    if (!it.single) total += it.blocks.size() == 1 ? kFrag : it.blocks.size() * kCFrag;
+  for (const PassPlan& pp : plan) total += pp.kind == 1 ? kFrag : pp.kind == 2 ? pp.blocks.size() * kCFrag : 0;
   YH_TRY(w.sync());
   YH_TRY(w.bpin.ensure(std::max<size_t>(1, total) * sizeof(R)));
   YH_TRY(w.tiles.ensure(std::max<size_t>(1, total) * sizeof(R)));
@@ -1007,13 +1073,13 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
   V* s = static_cast<V*>(state);
   std::vector<cd> U;
   size_t off = 0;
-  for (const PassItem& it : items) {
-    if (it.single) {
-      YH_TRY(single(bg[steps[it.step].gate]));
+  for (const PassPlan& pp : plan) {
+    if (pp.kind == 0) {
+      YH_TRY(single(bg[steps[pp.step].gate]));
       continue;
     }
-    if (it.blocks.size() == 1) {
-      const BStep& st = steps[it.blocks[0]];
+    if (pp.kind == 1) {
+      const BStep& st = steps[pp.step];
       block_matrix(st, bg, U);
       to_fragments<R>(U, hst + off);
       YH_HIP(hipMemcpyAsync(dev + off, hst + off, kFrag * sizeof(R), hipMemcpyHostToDevice, w.stream));
@@ -1035,54 +1101,17 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       off += kFrag;
       continue;
     }
-    // a multi-block pass: tile bits = the pass's bits padded with the lowest free
-    // bits to exactly 12
     TBlockArgs a{};
-    uint64_t tb = it.bits;
-    for (int b = 0; popc(tb) < kTT && b < nbits; ++b) tb |= uint64_t(1) << b;
-    if (popc(tb) != kTT || nbits <= kTT) return fail(YH_ERR_UNSUPPORTED, "block pass tile does not fit");
     int pos_of[64];
     for (int b = 0, k = 0; b < nbits; ++b)
-      if ((tb >> b) & 1) {
+      if ((pp.tile >> b) & 1) {
         a.tbit[k] = uint8_t(b);
         pos_of[b] = k++;
       }
-    a.nblk = int(it.blocks.size());
-    if (a.nblk < 2 || a.nblk > kPassMax) return fail(YH_ERR_UNSUPPORTED, "block pass size out of range");
+    a.nblk = int(pp.blocks.size());
     a.ntiles = uint64_t(1) << (nbits - kTT);
-    std::vector<BStep> ps;  // the pass's blocks with their identity rows in the tile
-    for (size_t k : it.blocks) {
-      ps.push_back(steps[k]);
-      pad_in_tile(ps.back(), tb);
-    }
-    // A block whose rows avoid memory bits 0..3 can take its operands from HBM
-    // (first) or put its results to HBM (last) with whole 16-amplitude runs; such a
-    // block is moved to the front / the end when it acts on no bit of the blocks it
-    // moves past.
-    std::vector<size_t> order(ps.size());
-    for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-    const uint64_t low = (uint64_t(1) << kTLow) - 1;
-    auto rows = [&](size_t k) { return step_mask(ps[order[k]]); };
-    auto mask = [&](size_t k) { return ps[order[k]].support; };
-    bool din = false, dout = false;
-    for (size_t j = 0; j < order.size() && !din; ++j) {
-      bool ok = !(rows(j) & low);
-      for (size_t k = 0; k < j && ok; ++k) ok = !(mask(j) & mask(k));
-      if (ok) {
-        std::rotate(order.begin(), order.begin() + j, order.begin() + j + 1);
-        din = true;
-      }
-    }
-    for (size_t j = order.size(); j-- > (din ? 1u : 0u) && !dout;) {
-      bool ok = !(rows(j) & low);
-      for (size_t k = j + 1; k < order.size() && ok; ++k) ok = !(mask(j) & mask(k));
-      if (ok) {
-        std::rotate(order.begin() + j, order.begin() + j + 1, order.end());
-        dout = true;
-      }
-    }
-    for (size_t bi = 0; bi < order.size(); ++bi) {
-      const BStep& st = ps[order[bi]];
+    for (size_t bi = 0; bi < pp.blocks.size(); ++bi) {
+      const BStep& st = pp.blocks[bi];
       const int nq = st.small ? kBQ - 1 : kBQ;
       int qs[kBQ];
       for (int j = 0; j < kBQ; ++j) {
@@ -1099,7 +1128,8 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
         to_compact<R>(U, hst + off + bi * kCFrag);
       }
     }
-    const size_t sz = it.blocks.size() * kCFrag;
+    const bool din = pp.din, dout = pp.dout;
+    const size_t sz = pp.blocks.size() * kCFrag;
     YH_HIP(hipMemcpyAsync(dev + off, hst + off, sz * sizeof(R), hipMemcpyHostToDevice, w.stream));
     const uint64_t grid = std::min<uint64_t>(a.ntiles, uint64_t(res[1]));
     YH_TRY(w.time_begin(YH_K_TBLOCK));
@@ -1180,6 +1210,49 @@ int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]) {
       out[1] += int64_t(s.members.size());
     } else {
       out[2] += 1;
+    }
+  }
+  return YH_OK;
+}
+
+// Host-only launch plan of the block path. dm = 0: the state-vector plan of a
+// circuit on nbits qubits; dm = 1: the superoperator plan of a density matrix of
+// nbits qubits. out[0] launches of the block kernels (k_block5 + k_tblocks),
+// out[1] multi-block passes, out[2] blocks in them, out[3] of those the 4-bit ones,
+// out[4] multi-block passes with direct-in, out[5] with direct-out, out[6] gates
+// run as their own launch.
+int block_pass_stats(int nbits, const yh_op* ops, size_t nops, int dm, int64_t out[7]) {
+  std::vector<BStep> steps;
+  std::vector<BGate> bg;
+  int nb = nbits;
+  for (int k = 0; k < 7; ++k) out[k] = 0;
+  if (dm) {
+    YH_TRY(to_bgates_dm(nbits, ops, nops, bg));
+    nb = 2 * nbits;
+    if (bg.empty()) return YH_OK;
+    if (nb < kBQ) {
+      out[6] = int64_t(bg.size());
+      return YH_OK;
+    }
+    YH_TRY(plan_gates(bg, nb, structure_key(nb, ops, nops) ^ 0x9e3779b97f4a7c15ull, nops, steps));
+  } else {
+    if (nbits < kBQ) return fail(YH_ERR_UNSUPPORTED, "fewer qubits than a block");
+    YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));
+  }
+  std::vector<PassPlan> plan;
+  YH_TRY(plan_passes(steps, nb, plan));
+  for (const PassPlan& pp : plan) {
+    if (pp.kind == 0) {
+      out[6] += 1;
+      continue;
+    }
+    out[0] += 1;
+    if (pp.kind == 2) {
+      out[1] += 1;
+      out[2] += int64_t(pp.blocks.size());
+      for (const BStep& st : pp.blocks) out[3] += st.small;
+      out[4] += pp.din;
+      out[5] += pp.dout;
     }
   }
   return YH_OK;

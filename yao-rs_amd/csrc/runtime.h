@@ -78,6 +78,7 @@ template <class R>
 int apply_blocks(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);
 int block_plan_stats(int nbits, const yh_op* ops, size_t nops, int64_t out[3]);
 int block_plan_bits(int nbits, const yh_op* ops, size_t nops, int32_t* bits, size_t cap, size_t* nsteps);
+int block_pass_stats(int nbits, const yh_op* ops, size_t nops, int dm, int64_t out[7]);
 // Density matrix (2n-bit vector, row bits above column bits) through the block
 // path: gates and channels as superoperators; an element on more than 5 bits goes
 // to single_op(op index).

@@ -138,6 +138,7 @@ SIGNATURES = [
                                P(ctypes.c_int32), P(ctypes.c_int32), P(c_double), P(c_double), c_size_t]),
     ("yh_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
     ("yh_block_plan_stats", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int64)]),
+    ("yh_block_pass_stats", c_int, [c_int, P(YhOp), c_size_t, c_int, P(ctypes.c_int64)]),
     ("yh_block_plan", c_int, [c_int, P(YhOp), c_size_t, P(ctypes.c_int32), c_size_t, P(c_size_t)]),
     ("yh_gate_matrix", c_int, [P(YhOp), P(c_double), P(c_int)]),
     ("yh_expect", c_int, [c_void_p, c_size_t, P(c_double), P(ctypes.c_int32),
