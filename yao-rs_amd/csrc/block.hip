@@ -280,14 +280,18 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
     g.in_lane = roff(g, grp);
     g.out_lane = roff(g, M::drow(lane, 0));
     // this lane's column(s), then their tile positions
-    int cb = g.small ? wave * 2 * kCols + col : wave * kCols + col, cb1 = cb + kCols;
+    auto at = [&](int c) {
 #pragma unroll
-    for (int j = 0; j < kBQ; ++j) {
-      cb = ((cb >> qs[j]) << (qs[j] + 1)) | (cb & ((1 << qs[j]) - 1));
-      cb1 = ((cb1 >> qs[j]) << (qs[j] + 1)) | (cb1 & ((1 << qs[j]) - 1));
+      for (int j = 0; j < kBQ; ++j) c = ((c >> qs[j]) << (qs[j] + 1)) | (c & ((1 << qs[j]) - 1));
+      return c;
+    };
+    if (g.small) {
+      g.cb = at(wave * 2 * kCols + col);
+      g.cb1 = at(wave * 2 * kCols + kCols + col);
+    } else {
+      g.cb = at(wave * kCols + col);
+      g.cb1 = 0;
     }
-    g.cb = cb;
-    g.cb1 = cb1;
   };
   // tile position of operand m (B fragment: k-step m, or group m / 4 and k-step
   // m % 4) and of result m (row tile m / 4, or group m / 4; element m % 4)
@@ -359,7 +363,11 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
         for (int gi = 0; gi < 2; ++gi) {
-          const V xv = x[gi * 4 + ks];
+          V xv = x[gi * 4 + ks];
+          // an empty asm keeps this path's operand sums from being merged with the
+          // other path's and hoisted above the branch (with every wait for the
+          // prefetched operands of a direct-in block)
+          asm volatile("" : "+v"(xv.x));
           p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
           p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
           p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
