@@ -153,21 +153,31 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None, "kernel": dom,
                 "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(kstats[dom]["avg_us"], 2)}
 
-    def mfma_of(kstats, peak, nblocks, steps):
-        # real-split 64 x 64 product per 32 amplitudes: 64 * 64 * 2 flops per 32
-        # amplitudes and block; blocks run in k_block5 (one per launch) and k_tblocks
-        # (several per launch), so the rate is over both kernels' time
-        flops = (1 << n) * 64 * 64 * 2 // 32
+    # algorithmic flops of one block: a 32 x 32 complex matrix on every 32 amplitudes
+    # (32 complex multiply-adds of 8 real flops per amplitude), or 16 x 16 on every 16
+    F5 = (1 << n) * 32 * 8
+    F4 = (1 << n) * 16 * 8
+
+    def mfma_of(kstats, peak, steps):
+        # blocks run in k_block5 (one per launch) and k_tblocks (several per launch),
+        # so the rate is over both kernels' time
+        flops = (blocks5 * F5 + blocks4 * F4) * steps
         t = sum(kstats[k]["total_ms"] for k in ("fused", "tblock") if k in kstats) * 1e-3
-        rate = flops * nblocks * steps / t
-        return {"flops_per_block": flops, "blocks_per_step": nblocks, "achieved_tflops": round(rate / 1e12, 2),
-                "peak_tflops": peak, "frac": round(rate / 1e12 / peak, 4)}
+        rate = flops / t
+        return {"flops_per_block": {"32x32": F5, "16x16": F4}, "blocks_per_step": {"32x32": blocks5, "16x16": blocks4},
+                "achieved_tflops": round(rate / 1e12, 2), "peak_tflops": peak, "frac": round(rate / 1e12 / peak, 4)}
 
     gbg_dt, gbg_k = run(0, 1, 0 if args.warmup == 0 else 1, "gate-by-gate")
     tile_dt, tile_k = run(4, 1, 0 if args.warmup == 0 else 1, "tiled")
     dt, kstats = run(5, args.steps, args.warmup, "blocks")
     bplan = np.zeros(3, dtype=np.int64)
     A.check(L.yh_block_plan_stats(n, ol.arr, ol.n, A.i64ptr(bplan)))
+    # the launch plan: passes, multi-block passes, blocks in them, 16 x 16 ones among them
+    pst = np.zeros(7, dtype=np.int64)
+    A.check(L.yh_block_pass_stats(n, ol.arr, ol.n, 0, A.i64ptr(pst)))
+    passes, multi, in_multi, small = (int(x) for x in pst[:4])
+    blocks4 = small
+    blocks5 = (passes - multi) + (in_multi - small)
     value = ngates * args.steps / dt
     ms_per_step = 1e3 * dt / args.steps
     # The two block kernels: k_block5 (a block alone in its HBM pass, HBM-bound)
@@ -180,7 +190,6 @@ def main():
     def block_rooflines(ks, amp_bytes, peak_tflops, steps):
         """k_block5 (one block per launch) as HBM-bound, k_tblocks (several per
         launch) as MFMA-bound, from the launches' HIP-event averages."""
-        flops_block = (1 << n) * 64 * 64 * 2 // 32
         traffic = TRAFFIC28_BYTES.get(amp_bytes, {}) if n == 28 else {}
         per = {}
         if "fused" in ks:
@@ -191,15 +200,17 @@ def main():
             per["k_block5"] = r
         if "tblock" in ks:
             t = ks["tblock"]
-            bpl = (int(bplan[0]) * steps - ks.get("fused", {}).get("launches", 0)) / t["launches"]
-            tf = bpl * flops_block / (t["avg_us"] * 1e-6) / 1e12
+            assert t["launches"] == multi * steps, (t["launches"], multi, steps)
+            fpl = ((in_multi - small) * F5 + small * F4) / multi
+            tf = fpl / (t["avg_us"] * 1e-6) / 1e12
             per["k_tblocks"] = {
                 "bound": "mfma", "achieved": round(tf, 2), "peak": peak_tflops, "unit": "TFLOP/s",
                 "frac": round(tf / peak_tflops, 4), "traffic": traffic.get("tblock"), "kernel": "tblock",
-                "blocks_per_launch": round(bpl, 3), "flops_per_launch": int(bpl * flops_block),
+                "blocks_per_launch": round(in_multi / multi, 3),
+                "blocks_16x16_per_launch": round(small / multi, 3), "flops_per_launch": int(fpl),
                 # the kernel forms each complex product from three real ones (Gauss),
                 # so the MFMA units execute 3/4 of the algorithmic flops
-                "mfma_flops_executed_per_launch": int(bpl * flops_block * 3 // 4),
+                "mfma_flops_executed_per_launch": int(fpl * 3 // 4),
                 "avg_launch_us": round(t["avg_us"], 2),
                 "hbm_gbps": round(2 * amp_bytes * (1 << n) / (t["avg_us"] * 1e-6) / 1e9, 1)}
         return per
@@ -213,14 +224,16 @@ def main():
         "value": round(value, 2), "unit": "gate-apps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"random brickwork {n}q depth {depth} complex128 from |0...0>, 5-qubit fused "
-                               f"blocks applied with FP64 MFMA (yh_apply_fused, max_fuse_qubits=5)",
-                   "nbits": n, "depth": depth, "gates": ngates, "blocks": int(bplan[0])},
-        "hbm_gbps_effective": round(int(bplan[0]) * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
+        "config": {"workload": f"random brickwork {n}q depth {depth} complex128 from |0...0>, gates fused "
+                               f"into 4- and 5-qubit blocks applied with FP64 MFMA (yh_apply_fused, "
+                               f"max_fuse_qubits=5)",
+                   "nbits": n, "depth": depth, "gates": ngates, "blocks": int(bplan[0]),
+                   "blocks_16x16": blocks4, "hbm_passes": passes},
+        "hbm_gbps_effective": round(passes * 2 * 16 * (1 << n) * args.steps / dt / 1e9, 1),
         "kernels": {k: {kk: round(vv, 3) for kk, vv in v.items()} for k, v in kstats.items()},
         "roofline": roof,
         "roofline_kernels": per_kernel,
-        "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS, int(bplan[0]), args.steps),
+        "mfma": mfma_of(kstats, FP64_MFMA_PEAK_TFLOPS, args.steps),
         "tiled": {"value": round(ngates / tile_dt, 2), "unit": "gate-apps/s", "ms_per_step": round(1e3 * tile_dt, 3),
                   "tile_passes": int(plan[0]), "gate_groups": int(plan[1]),
                   "hbm_gbps_effective": round(int(plan[0]) * 2 * 16 * (1 << n) / tile_dt / 1e9, 1),
@@ -262,7 +275,7 @@ def main():
                             "ms_per_step": round(1e3 * dt64 / steps64, 3),
                             "max_rel_err_vs_fp64": err64, "fidelity_vs_fp64": fid64,
                             "roofline_kernels": block_rooflines(k64, 8, FP32_MFMA_PEAK_TFLOPS, steps64),
-                            "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS, int(bplan[0]), steps64),
+                            "mfma": mfma_of(k64, FP32_MFMA_PEAK_TFLOPS, steps64),
                             "dtype": "f32 amplitudes and FP32 MFMA, block matrices fused in f64 on the host"}
         log(f"complex64: {1e3 * dt64 / steps64:.2f} ms/step, max rel err {err64:.3e}, fidelity {fid64:.12f}")
         del reg64, reg64b, got64, ref

@@ -146,8 +146,9 @@ def test_tile_plan_covers_every_gate():
 
 
 def test_block_plan_covers_every_gate():
-    """Host-only 5-qubit block planner on the 28-qubit depth-20 brickwork: every
-    gate lands in a block; far fewer blocks than gates. Wide gates run alone."""
+    """Host-only block planner on the 28-qubit depth-20 brickwork: every gate lands
+    in a block; far fewer blocks than gates (blocks of 4 or 5 bits). Wide gates run
+    alone."""
     from fixture_circuits import brickwork
     els = brickwork(28, 20, 20)
     ol = lower(els)
@@ -155,7 +156,7 @@ def test_block_plan_covers_every_gate():
     A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
     blocks, inblk, single = [int(x) for x in out]
     assert inblk == len(els) and single == 0
-    assert blocks <= 50
+    assert blocks <= 80
     wide = [PositionedGate(Gate.Custom(np.eye(8), False), [0, 1, 2], [3, 4, 5], [True, True, False])]
     ol = lower(els[:10] + wide)
     A.check(A.lib().yh_block_plan_stats(28, ol.arr, ol.n, A.i64ptr(out)))
@@ -200,20 +201,32 @@ def _pass_stats(n, els, dm):
     return list(out)
 
 
+def _modelled(stats):
+    """The planner's launch-time model, in 32 x 32 block-times: 0.46 per multi-block
+    pass, 1 per 32 x 32 and 0.55 per 16 x 16 block in it, 1.8 per block alone, 1 per
+    gate alone."""
+    launches, multi, in_multi, small, din, dout, alone = stats
+    return 0.46 * multi + (in_multi - small) + 0.55 * small + 1.8 * (launches - multi) + alone
+
+
 def test_block_pass_stats_brickwork28():
-    """The 28-qubit depth-20 brickwork's launch plan: 48 blocks in 23 HBM passes,
-    20 of them multi-block; every block is in exactly one pass; most multi-block
-    passes read (first block) and write (last block) HBM directly."""
+    """The 28-qubit depth-20 brickwork's launch plan: every block is in exactly one
+    pass; the plan is mostly 16 x 16 blocks (their time per gate is lower), in at
+    most 20 HBM passes that read (first block) and write (last block) HBM directly;
+    its modelled time is below that of the plan of 48 blocks of 5 bits in 23
+    passes (57.8 block-times)."""
     from fixture_circuits import brickwork
     els = brickwork(28, 20, 20)
-    launches, multi, in_multi, small, din, dout, alone = _pass_stats(28, els, 0)
+    st = _pass_stats(28, els, 0)
+    launches, multi, in_multi, small, din, dout, alone = st
     plan = (ctypes.c_int64 * 3)()
     c = Circuit.qubits(28, els)
     A.check(A.lib().yh_block_plan_stats(28, c.ops.arr, c.ops.n, plan))
-    assert plan[0] == 48 and alone == 0
+    assert alone == 0
     assert (launches - multi) + in_multi == plan[0]  # blocks alone + blocks in passes
-    assert launches <= 23 and multi >= 20
-    assert din >= 16 and dout >= 12 and small <= in_multi
+    assert launches <= 20 and small >= in_multi // 2
+    assert din >= multi - 2 and dout >= multi // 2
+    assert _modelled(st) < 50
 
 
 def test_block_pass_stats_density_matrix():

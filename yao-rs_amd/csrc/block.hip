@@ -215,6 +215,8 @@ constexpr int kCFrag = 32 * 64;        // compact A values of one block
 // kNoBit, a position above every tile bit (inserting a zero there, or setting it
 // for row bit 4 of a row index below 16, changes nothing).
 constexpr int kNoBit = 30;
+// modelled time of a 16 x 16 block in a multi-block pass, in 32 x 32 block-times
+constexpr double kSmallCost = 0.55;
 
 struct TBlockArgs {
   uint8_t tbit[kTT];          // memory bit of tile position k, ascending (0..3 first)
@@ -573,11 +575,12 @@ struct Planner {
       taken.push_back(i);
     }
   }
+  int lim = kBQ;  // bits of the block being chosen (4 or 5)
   uint64_t first_come() const {
     uint64_t Q = 0, B = 0;
     for (uint32_t i : rem) {
       const BGate& g = bg[i];
-      if (!g.blockable || (g.mask & B) || popc(Q | g.mask) > kBQ)
+      if (!g.blockable || (g.mask & B) || popc(Q | g.mask) > lim)
         B |= g.mask;
       else
         Q |= g.mask;
@@ -595,7 +598,7 @@ struct Planner {
     }
   }
   uint64_t grow(uint64_t Q, std::vector<uint32_t>& tmp) const {
-    while (popc(Q) < kBQ) {
+    while (popc(Q) < lim) {
       absorb(Q, tmp);
       std::vector<char> in_blk;
       uint64_t cand = 0;
@@ -607,7 +610,7 @@ struct Planner {
           continue;
         }
         const uint64_t m = bg[i].mask;
-        if ((m & Q) && popc(m | Q) <= kBQ && (m & ~Q & ~cand)) {
+        if ((m & Q) && popc(m | Q) <= lim && (m & ~Q & ~cand)) {
           cand |= m & ~Q;
           nc = popc(cand);
           if (nc >= 8) break;
@@ -628,10 +631,33 @@ struct Planner {
     }
     return Q;
   }
-  void run(int strategy, std::vector<BStep>& steps) {
+  // the candidate block of at most `lim` bits that absorbs the most gates
+  void best_block(int strategy, std::vector<uint32_t>& best, uint64_t& bestQ) {
+    std::vector<uint32_t> taken, tmp, rd;
+    std::vector<uint64_t> cands{first_come()};
+    if (strategy == 1 || strategy == 3)
+      for (int b = 0; b + lim <= nbits; ++b) cands.push_back(((uint64_t(1) << lim) - 1) << b);
+    if (strategy == 2 || strategy == 3) {
+      ready(rd, 16);
+      for (uint32_t r : rd) cands.push_back(grow(bg[r].mask, tmp));
+    }
+    bestQ = 0;
+    best.clear();
+    for (uint64_t Q : cands) {
+      absorb(Q, taken);
+      if (taken.size() > best.size()) {
+        best.swap(taken);
+        bestQ = Q;
+      }
+    }
+  }
+  // small4: also try blocks of 4 bits (16 x 16, about half the time of a 5-bit
+  // block in a multi-block pass) and keep one when it absorbs more gates per unit
+  // of time
+  void run(int strategy, bool small4, std::vector<BStep>& steps) {
     rem.resize(bg.size());
     for (size_t i = 0; i < bg.size(); ++i) rem[i] = uint32_t(i);
-    std::vector<uint32_t> taken, best, tmp, rd;
+    std::vector<uint32_t> best, best4;
     std::vector<char> done(bg.size(), 0);
     while (!rem.empty()) {
       const uint32_t head = rem.front();
@@ -641,20 +667,16 @@ struct Planner {
         steps.push_back(s);
         done[head] = 1;
       } else {
-        std::vector<uint64_t> cands{first_come()};
-        if (strategy == 1 || strategy == 3)
-          for (int b = 0; b + kBQ <= nbits; ++b) cands.push_back(((uint64_t(1) << kBQ) - 1) << b);
-        if (strategy == 2 || strategy == 3) {
-          ready(rd, 16);
-          for (uint32_t r : rd) cands.push_back(grow(bg[r].mask, tmp));
-        }
-        uint64_t bestQ = 0;
-        best.clear();
-        for (uint64_t Q : cands) {
-          absorb(Q, taken);
-          if (taken.size() > best.size()) {
-            best.swap(taken);
-            bestQ = Q;
+        uint64_t bestQ = 0, bestQ4 = 0;
+        lim = kBQ;
+        best_block(strategy, best, bestQ);
+        if (small4 && popc(bg[head].mask) <= kBQ - 1) {
+          lim = kBQ - 1;
+          best_block(strategy, best4, bestQ4);
+          lim = kBQ;
+          if (!best4.empty() && double(best4.size()) > kSmallCost * double(best.size())) {
+            best.swap(best4);
+            bestQ = bestQ4;
           }
         }
         BStep s;
@@ -712,6 +734,8 @@ PlanCache& plan_cache() {
 
 // fewest-blocks plan of a gate list, cached by `key` (the structure of the ops the
 // gate list was built from)
+double plan_cost(const std::vector<BStep>& steps, int nbits);
+
 int plan_gates(const std::vector<BGate>& bg, int nbits, uint64_t key, size_t nops, std::vector<BStep>& steps) {
   {
     PlanCache& c = plan_cache();
@@ -722,19 +746,25 @@ int plan_gates(const std::vector<BGate>& bg, int nbits, uint64_t key, size_t nop
         return YH_OK;
       }
   }
+  // every strategy, with and without 4-bit blocks; the plan with the lowest
+  // modelled launch time is kept (ties: fewer blocks)
   Planner pl(bg, nbits);
   std::vector<BStep> bestp;
+  double best_cost = 1e300;
   size_t best_blocks = SIZE_MAX;
-  for (int strategy = 0; strategy < 4; ++strategy) {
-    std::vector<BStep> cand;
-    pl.run(strategy, cand);
-    size_t nb = 0;
-    for (auto& s : cand) nb += s.block;
-    if (nb < best_blocks) {
-      best_blocks = nb;
-      bestp.swap(cand);
+  for (int small4 = 0; small4 < 2; ++small4)
+    for (int strategy = 0; strategy < 4; ++strategy) {
+      std::vector<BStep> cand;
+      pl.run(strategy, small4 != 0, cand);
+      size_t nb = 0;
+      for (auto& s : cand) nb += s.block;
+      const double c = plan_cost(cand, nbits);
+      if (c < best_cost - 1e-9 || (c < best_cost + 1e-9 && nb < best_blocks)) {
+        best_cost = c;
+        best_blocks = nb;
+        bestp.swap(cand);
+      }
     }
-  }
   steps = bestp;
   PlanCache& c = plan_cache();
   std::lock_guard<std::mutex> lk(c.mu);
@@ -981,10 +1011,17 @@ struct PassPlan {
 int plan_passes(const std::vector<BStep>& steps, int nbits, std::vector<PassPlan>& out) {
   // of the two packings, the one with the lower modelled time: a pass of k >= 2
   // blocks costs about 0.46 + k block-times, a block alone in its pass 1.8
-  auto cost = [](const std::vector<PassItem>& v) {
+  auto cost = [&](const std::vector<PassItem>& v) {
     double c = 0;
-    for (const PassItem& it : v)
-      if (!it.single) c += it.blocks.size() == 1 ? 1.8 : 0.46 + double(it.blocks.size());
+    for (const PassItem& it : v) {
+      if (it.single) continue;
+      if (it.blocks.size() == 1) {
+        c += 1.8;
+        continue;
+      }
+      c += 0.46;
+      for (size_t k : it.blocks) c += popc(steps[k].support) <= kBQ - 1 ? kSmallCost : 1.0;
+    }
     return c;
   };
   std::vector<PassItem> items = group_passes(steps, nbits, false);
@@ -1042,6 +1079,24 @@ int plan_passes(const std::vector<BStep>& steps, int nbits, std::vector<PassPlan
     out.push_back(std::move(pp));
   }
   return YH_OK;
+}
+
+// modelled time of a plan's launches, in 32 x 32 block-times: a block alone in its
+// pass (k_block5, HBM-bound) 1.8, a multi-block pass 0.46 plus 1 per 32 x 32 and
+// kSmallCost per 16 x 16 block, a gate on its own path 1
+double plan_cost(const std::vector<BStep>& steps, int nbits) {
+  std::vector<PassPlan> plan;
+  if (plan_passes(steps, nbits, plan) != YH_OK) return 1e300;
+  double c = 0;
+  for (const PassPlan& pp : plan) {
+    if (pp.kind == 0) c += 1.0;
+    else if (pp.kind == 1) c += 1.8;
+    else {
+      c += 0.46;
+      for (const BStep& b : pp.blocks) c += b.small ? kSmallCost : 1.0;
+    }
+  }
+  return c;
 }
 
 }  // namespace
