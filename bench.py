@@ -30,15 +30,15 @@ HBM_PEAK_GBPS = 8000.0
 FP64_MFMA_PEAK_TFLOPS = 78.6
 FP32_MFMA_PEAK_TFLOPS = 157.3
 # HBM bytes per launch on the 28-qubit state, from rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 20 of k_tblocks
-# in its four direct-in/out forms, 3 of k_block5), each scaled by the factor a plain
-# copy of known size gives for the same counter (2 for FETCH_SIZE, 1 for
-# WRITE_SIZE); summary in profiles/k_block5_brickwork28_pmc.md. Keyed by the
-# amplitude size in bytes (16 complex128, 8 complex64).
-TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2097696 + 4194304) * 1024),
-                        "fused": int((2 * 2097495 + 4194304) * 1024)},
-                   8: {"tblock": int((2 * 1048740 + 2097152) * 1024),
-                       "fused": int((2 * 1048776 + 2097152) * 1024)}}
+# WRITE_SIZE (KiB per dispatch, averaged over one step's launches: 18 of k_tblocks,
+# 1 of k_block5), each scaled by the factor a plain copy of known size gives for the
+# same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
+# profiles/block_kernels_brickwork28_pmc.md. Keyed by the amplitude size in bytes
+# (16 complex128, 8 complex64).
+TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2098294 + 4194304) * 1024),
+                        "fused": int((2 * 2100862 + 5137207) * 1024)},
+                   8: {"tblock": int((2 * 1048827 + 2097152) * 1024),
+                       "fused": int((2 * 1083938 + 2351752) * 1024)}}
 NBITS = 28
 DEPTH = 20
 SEED = 20

@@ -150,15 +150,16 @@ int yh_validate(int nbits, const yh_op *ops, size_t nops);
 int yh_apply(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops);
 /* Same semantics as yh_apply (apply_inplace, apply.rs:123-147), with gates packed
  * into passes that each stream the state through HBM once.
- * max_fuse_qubits = 5 or 0 (default): gates fused into 5-qubit blocks (32 x 32
- *   complex matrices, FP64 on the host), each applied as an MFMA contraction
- *   (FP64 MFMA for complex128, FP32 MFMA for complex64). The planner keeps the
- *   fewest-blocks plan of several strategies and caches it by circuit structure
- *   (gate kinds and positions), so repeated calls only rebuild the block
- *   matrices. Blocks whose bits fit a 12-bit tile together with memory bits 0..3
- *   share one HBM pass (up to 5 blocks applied from LDS, each complex product
- *   formed from three real ones); a block that fits with no other streams through
- *   HBM on its own (real-split product, four real ones).
+ * max_fuse_qubits = 5 or 0 (default): gates fused into blocks of up to 5 qubits
+ *   (32 x 32 complex matrices, or 16 x 16 for blocks on at most 4 qubits; FP64 on
+ *   the host), each applied as an MFMA contraction (FP64 MFMA for complex128,
+ *   FP32 MFMA for complex64). The planner keeps the plan with the lowest modelled
+ *   launch time of several strategies and caches it by circuit structure (gate
+ *   kinds and positions), so repeated calls only rebuild the block matrices.
+ *   Blocks whose bits fit a 12-bit tile together with memory bits 0..3 share one
+ *   HBM pass (up to 5 blocks applied from LDS, each complex product formed from
+ *   three real ones); a block that fits with no other streams through HBM on its
+ *   own (32 x 32, real-split product, four real ones).
  * max_fuse_qubits in 1..4: multi-gate LDS tile passes (12 qubits per pass, gates
  *   applied in groups of up to 4 qubits). */
 int yh_apply_fused(yh_reg *reg, int circuit_nbits, const yh_op *ops, size_t nops,
