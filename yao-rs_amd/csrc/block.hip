@@ -229,13 +229,19 @@ struct TBlockArgs {
 
 __device__ __forceinline__ int tpad(int p) { return p + (p >> 4); }
 
-template <class R, bool DIN, bool DOUT>
+// NTL = 2: the pass's blocks are all 16 x 16 and each step of a workgroup takes two
+// tiles (adjacent tile indices) in LDS at once, so each block's geometry, A values
+// and barrier serve twice the MFMA work; their A values are staged kSFrag apart.
+constexpr int kSFrag = 8 * 64;  // compact A values of one 16 x 16 block
+
+template <class R, bool DIN, bool DOUT, int NTL>
 __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* __restrict__ s, TBlockArgs a,
                                                         const R* __restrict__ frag) {
   using V = typename Cplx<R>::V;
   using M = MfmaOps<R>;
-  __shared__ V lds[kTN + kTN / 16];
-  __shared__ R afr[kPassMax * kCFrag];
+  constexpr int kAStride = NTL == 2 ? kSFrag : kCFrag;
+  __shared__ V lds[NTL][kTN + kTN / 16];
+  __shared__ R afr[kPassMax * kAStride];
   __shared__ uint64_t hoff[1 << kTHigh];
   __shared__ int bq[kPassMax][2 * kBQ];  // per block: q[0..4], then qs[0..4]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -252,7 +258,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
     bq[b][j] = a.q[b][j];
     bq[b][kBQ + j] = a.qs[b][j];
   }
-  for (int e = tid; e < a.nblk * kCFrag; e += kTThreads) afr[e] = frag[e];
+  for (int e = tid; e < a.nblk * kAStride; e += kTThreads) afr[e] = frag[e];
   __syncthreads();
   constexpr int kPer = kTN / kTThreads;
   // memory offset (from the tile's base) of tile position p
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       g.q[j] = bq[b][j];
       qs[j] = bq[b][kBQ + j];
     }
-    g.small = (a.small >> b) & 1;
+    g.small = NTL == 2 || ((a.small >> b) & 1);
     g.in_lane = roff(g, grp);
     g.out_lane = roff(g, M::drow(lane, 0));
     // this lane's column(s), then their tile positions
@@ -304,156 +310,169 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
     const int d = M::drow(lane, m & 3) - M::drow(lane, 0);
     return g.small ? ((m >> 2 ? g.cb1 : g.cb) | g.out_lane | roff(g, d)) : (g.cb | g.out_lane | roff(g, 16 * (m >> 2) + d));
   };
-  uint64_t t = blockIdx.x;
-  if (t >= a.ntiles) return;  // uniform
-  uint64_t base = insert_zeros(t, a.tbit, kTT);
+  // a workgroup step takes NTL adjacent tiles (a "unit"); units are dealt out in
+  // turn to the persistent workgroups
+  const uint64_t nunits = a.ntiles / NTL;
+  uint64_t u = blockIdx.x;
+  if (u >= nunits) return;  // uniform
+  auto unit_base = [&](uint64_t uu, int i) { return insert_zeros(uu * NTL + i, a.tbit, kTT); };
   static_assert(kPer == 8, "eight amplitudes per thread");
   Geo g0;
   geo(0, g0);
-  // the next tile's amplitudes: with DIN in block 0's MFMA operand layout, else in
+  // the next unit's amplitudes: with DIN in block 0's MFMA operand layout, else in
   // thread order (position tid + kTThreads k)
-  R vx[8], vy[8];
-  auto load_tile = [&](uint64_t bb) __attribute__((always_inline)) {
+  R vx[NTL][8], vy[NTL][8];
+  auto load_unit = [&](uint64_t uu) __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int p = DIN ? xpos(g0, k) : tid + kTThreads * k;
-      const V x = s[bb + mem(p)];
-      vx[k] = x.x;
-      vy[k] = x.y;
+    for (int i = 0; i < NTL; ++i) {
+      const uint64_t bb = unit_base(uu, i);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int p = DIN ? xpos(g0, k) : tid + kTThreads * k;
+        const V x = s[bb + mem(p)];
+        vx[i][k] = x.x;
+        vy[i][k] = x.y;
+      }
     }
   };
   auto fill = [&]() __attribute__((always_inline)) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) lds[tpad(tid + kTThreads * k)] = cmk<V>(vx[k], vy[k]);
+    for (int i = 0; i < NTL; ++i)
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) lds[i][tpad(tid + kTThreads * k)] = cmk<V>(vx[i][k], vy[i][k]);
   };
-  // block b of the pass on the tile at tb; `first` (DIN) takes the operands from
-  // vx/vy and then loads the tile at bn into them, `to_hbm` (DOUT) writes the
-  // results to HBM instead of LDS. Both are constants where the block is inlined.
-  auto block = [&](int b, bool first, bool to_hbm, uint64_t tb, uint64_t bn) __attribute__((always_inline)) {
+  // block b of the pass on the unit uc; `first` (DIN) takes the operands from
+  // vx/vy and then loads the unit un into them, `to_hbm` (DOUT) writes the results
+  // to HBM instead of LDS. Both are constants where the block is inlined.
+  auto block = [&](int b, bool first, bool to_hbm, uint64_t uc, uint64_t un) __attribute__((always_inline)) {
     // the block's geometry first (its LDS reads return in order, so they go
-    // before the A values'), then the operands' reads, then the A values
+    // before the A values'), then per tile the operands, the MFMAs and the results
     Geo g;
     if (b == 0) g = g0;
     else geo(b, g);
-    V x[8];
-    if (DIN && first) {
+    const R* f = afr + b * kAStride;
 #pragma unroll
-      for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[m], vy[m]);
-    } else {
+    for (int i = 0; i < NTL; ++i) {
+      V x[8];
+      if (DIN && first) {
 #pragma unroll
-      for (int m = 0; m < 8; ++m) x[m] = lds[tpad(xpos(g, m))];
-    }
-    const R* f = afr + b * kCFrag;
-    // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
-    // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2). Accumulator
-    // pair index: row tile (5-bit block) or column group (4-bit block).
-    typename M::acc_t p1[2], p2[2], p3[2];
+        for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[i][m], vy[i][m]);
+      } else {
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt) {
-      p1[rt] = typename M::acc_t{0, 0, 0, 0};
-      p2[rt] = typename M::acc_t{0, 0, 0, 0};
-      p3[rt] = typename M::acc_t{0, 0, 0, 0};
-    }
-    if (g.small) {
-      R ur[4], ui[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        ur[e] = f[e * 64 + lane];
-        ui[e] = f[(4 + e) * 64 + lane];
+        for (int m = 0; m < 8; ++m) x[m] = lds[i][tpad(xpos(g, m))];
       }
+      // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
+      // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2). Accumulator
+      // pair index: row tile (5-bit block) or column group (4-bit block).
+      typename M::acc_t p1[2], p2[2], p3[2];
 #pragma unroll
-      for (int ks = 0; ks < 4; ++ks)
+      for (int rt = 0; rt < 2; ++rt) {
+        p1[rt] = typename M::acc_t{0, 0, 0, 0};
+        p2[rt] = typename M::acc_t{0, 0, 0, 0};
+        p3[rt] = typename M::acc_t{0, 0, 0, 0};
+      }
+      if (g.small) {
+        R ur[4], ui[4];
 #pragma unroll
-        for (int gi = 0; gi < 2; ++gi) {
-          V xv = x[gi * 4 + ks];
-          // an empty asm keeps this path's operand sums from being merged with the
-          // other path's and hoisted above the branch (with every wait for the
-          // prefetched operands of a direct-in block)
-          asm volatile("" : "+v"(xv.x));
-          p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
-          p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
-          p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
+        for (int e = 0; e < 4; ++e) {
+          ur[e] = f[e * 64 + lane];
+          ui[e] = f[(4 + e) * 64 + lane];
         }
-    } else {
-      R ur[16], ui[16];
 #pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        ur[e] = f[e * 64 + lane];
-        ui[e] = f[(16 + e) * 64 + lane];
-      }
+        for (int ks = 0; ks < 4; ++ks)
 #pragma unroll
-      for (int ks = 0; ks < 8; ++ks) {
-        const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+          for (int gi = 0; gi < 2; ++gi) {
+            V xv = x[gi * 4 + ks];
+            // an empty asm keeps this path's operand sums from being merged with the
+            // other path's and hoisted above the branch (with every wait for the
+            // prefetched operands of a direct-in block)
+            asm volatile("" : "+v"(xv.x));
+            p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
+            p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
+            p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
+          }
+      } else {
+        R ur[16], ui[16];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt) {
-          p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
-          p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
-          p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+        for (int e = 0; e < 16; ++e) {
+          ur[e] = f[e * 64 + lane];
+          ui[e] = f[(16 + e) * 64 + lane];
+        }
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+          const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+#pragma unroll
+          for (int rt = 0; rt < 2; ++rt) {
+            p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
+            p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
+            p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+          }
         }
       }
-    }
-    if (DIN && first) {
-      // the operands are consumed: the next tile's loads reuse their registers
-      // (kept below the MFMAs, so no second register set is needed)
-      __builtin_amdgcn_sched_barrier(0);
-      load_tile(bn);
-    }
+      if (DIN && first && i == NTL - 1) {
+        // every tile's operands are consumed: the next unit's loads reuse their
+        // registers (kept below the MFMAs, so no second register set is needed)
+        __builtin_amdgcn_sched_barrier(0);
+        load_unit(un);
+      }
+      const uint64_t tb = to_hbm ? unit_base(uc, i) : 0;
 #pragma unroll
-    for (int m = 0; m < 8; ++m) {
-      V o;
-      const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
-      o.x = a1 - a2;
-      o.y = p3[m >> 2][m & 3] - a1 - a2;
-      const int p = opos(g, m);
-      if (to_hbm) s[tb + mem(p)] = o;
-      else lds[tpad(p)] = o;
+      for (int m = 0; m < 8; ++m) {
+        V o;
+        const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
+        o.x = a1 - a2;
+        o.y = p3[m >> 2][m & 3] - a1 - a2;
+        const int p = opos(g, m);
+        if (to_hbm) s[tb + mem(p)] = o;
+        else lds[i][tpad(p)] = o;
+      }
     }
     lds_barrier();
   };
-  // One tile: the next tile's loads are issued before this tile's stores, and every
+  // One unit: the next unit's loads are issued before this unit's stores, and every
   // wait on them follows those stores in straight-line code (the fill, or the next
-  // tile's first block after the peeled first tile), so the stores stay in flight
-  // while the next tile starts. A pass holds at least two blocks (the host runs a
+  // unit's first block after the peeled first unit), so the stores stay in flight
+  // while the next unit starts. A pass holds at least two blocks (the host runs a
   // block alone with k_block5), so the first and the DOUT block differ.
-  auto run_tile = [&](uint64_t tb, uint64_t bn) __attribute__((always_inline)) {
-    if (!DIN) load_tile(bn);
+  auto run_unit = [&](uint64_t uc, uint64_t un) __attribute__((always_inline)) {
+    if (!DIN) load_unit(un);
     const int nmid = DOUT ? a.nblk - 1 : a.nblk;
-    block(0, true, false, tb, bn);
-    for (int b = 1; b < nmid; ++b) block(b, false, false, tb, bn);
-    if (DOUT) block(a.nblk - 1, false, true, tb, bn);
+    block(0, true, false, uc, un);
+    for (int b = 1; b < nmid; ++b) block(b, false, false, uc, un);
+    if (DOUT) block(a.nblk - 1, false, true, uc, un);
     if (!DOUT) {
       // without DIN each thread stores, then refills, the same LDS positions, so no
-      // barrier is needed between the two; with DIN the next tile's first block
+      // barrier is needed between the two; with DIN the next unit's first block
       // writes other positions, so a barrier follows the store
 #pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const int p = tid + kTThreads * k;
-        s[tb + mem(p)] = lds[tpad(p)];
+      for (int i = 0; i < NTL; ++i) {
+        const uint64_t tb = unit_base(uc, i);
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+          const int p = tid + kTThreads * k;
+          s[tb + mem(p)] = lds[i][tpad(p)];
+        }
       }
       if (DIN) lds_barrier();
     }
   };
-  // the last tile of a workgroup loads itself again (unused): unconditional loads
+  // the last unit of a workgroup loads itself again (unused): unconditional loads
   // keep the operands in registers
-  auto next_base = [&](uint64_t tn) { return tn < a.ntiles ? insert_zeros(tn, a.tbit, kTT) : base; };
-  load_tile(base);
+  load_unit(u);
   if (!DIN) {
     fill();
     lds_barrier();
   }
-  uint64_t tn = t + gridDim.x;
-  uint64_t bn = next_base(tn);
-  run_tile(base, bn);  // peeled: the loop is entered with the same operations in flight as at its back edge
-  while (tn < a.ntiles) {  // uniform
+  uint64_t un = u + gridDim.x < nunits ? u + gridDim.x : u;
+  run_unit(u, un);  // peeled: the loop is entered with the same operations in flight as at its back edge
+  while (u + gridDim.x < nunits) {  // uniform
     if (!DIN) {
       fill();
       lds_barrier();
     }
-    t = tn;
-    base = bn;
-    tn = t + gridDim.x;
-    bn = next_base(tn);
-    run_tile(base, bn);
+    u += gridDim.x;
+    un = u + gridDim.x < nunits ? u + gridDim.x : u;
+    run_unit(u, un);
   }
 }
 
@@ -1116,7 +1135,7 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
     YH_HIP(hipGetDevice(&dev));
     YH_HIP(hipGetDeviceProperties(&prop, dev));
     YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_block5<R>, 256, 0));
-    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_tblocks<R, false, false>, kTThreads, 0));
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu_t, k_tblocks<R, false, false, 2>, kTThreads, 0));
     res[0] = prop.multiProcessorCount * std::max(1, per_cu);
     res[1] = prop.multiProcessorCount * std::max(1, per_cu_t);
   }
@@ -1173,6 +1192,11 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       }
     a.nblk = int(pp.blocks.size());
     a.ntiles = uint64_t(1) << (nbits - kTT);
+    // a pass of 16 x 16 blocks only takes two tiles per workgroup step
+    bool all_small = true;
+    for (const BStep& st : pp.blocks) all_small = all_small && st.small;
+    const bool pair = all_small && a.ntiles >= 2;
+    const size_t stride = pair ? kSFrag : kCFrag;
     for (size_t bi = 0; bi < pp.blocks.size(); ++bi) {
       const BStep& st = pp.blocks[bi];
       const int nq = st.small ? kBQ - 1 : kBQ;
@@ -1186,28 +1210,30 @@ int run_block_plan(Workspace& w, void* state, int nbits, const std::vector<BStep
       block_matrix(st, bg, U);
       if (st.small) {
         a.small |= 1u << bi;
-        to_compact_small<R>(U, hst + off + bi * kCFrag);
+        to_compact_small<R>(U, hst + off + bi * stride);
       } else {
-        to_compact<R>(U, hst + off + bi * kCFrag);
+        to_compact<R>(U, hst + off + bi * stride);
       }
     }
     const bool din = pp.din, dout = pp.dout;
-    const size_t sz = pp.blocks.size() * kCFrag;
+    const size_t sz = pp.blocks.size() * stride;
     YH_HIP(hipMemcpyAsync(dev + off, hst + off, sz * sizeof(R), hipMemcpyHostToDevice, w.stream));
-    const uint64_t grid = std::min<uint64_t>(a.ntiles, uint64_t(res[1]));
+    const uint64_t grid = std::min<uint64_t>(a.ntiles / (pair ? 2 : 1), uint64_t(res[1]));
     YH_TRY(w.time_begin(YH_K_TBLOCK));
-    if (din && dout)
-      hipLaunchKernelGGL((k_tblocks<R, true, true>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
-                         dev + off);
-    else if (din)
-      hipLaunchKernelGGL((k_tblocks<R, true, false>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
-                         dev + off);
-    else if (dout)
-      hipLaunchKernelGGL((k_tblocks<R, false, true>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
-                         dev + off);
-    else
-      hipLaunchKernelGGL((k_tblocks<R, false, false>), dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a,
-                         dev + off);
+    auto launch = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(unsigned(grid)), dim3(kTThreads), 0, w.stream, s, a, dev + off);
+    };
+    if (pair) {
+      if (din && dout) launch(k_tblocks<R, true, true, 2>);
+      else if (din) launch(k_tblocks<R, true, false, 2>);
+      else if (dout) launch(k_tblocks<R, false, true, 2>);
+      else launch(k_tblocks<R, false, false, 2>);
+    } else {
+      if (din && dout) launch(k_tblocks<R, true, true, 1>);
+      else if (din) launch(k_tblocks<R, true, false, 1>);
+      else if (dout) launch(k_tblocks<R, false, true, 1>);
+      else launch(k_tblocks<R, false, false, 1>);
+    }
     YH_HIP(hipGetLastError());
     YH_TRY(w.time_end());
     off += sz;
