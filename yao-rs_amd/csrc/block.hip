@@ -351,9 +351,8 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
     if (b == 0) g = g0;
     else geo(b, g);
     const R* f = afr + b * kAStride;
-#pragma unroll
-    for (int i = 0; i < NTL; ++i) {
-      V x[8];
+    // tile i's operands: registers (DIN, first block) or LDS
+    auto operands = [&](int i, V (&x)[8]) __attribute__((always_inline)) {
       if (DIN && first) {
 #pragma unroll
         for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[i][m], vy[i][m]);
@@ -361,54 +360,12 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
 #pragma unroll
         for (int m = 0; m < 8; ++m) x[m] = lds[i][tpad(xpos(g, m))];
       }
-      // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
-      // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2). Accumulator
-      // pair index: row tile (5-bit block) or column group (4-bit block).
-      typename M::acc_t p1[2], p2[2], p3[2];
-#pragma unroll
-      for (int rt = 0; rt < 2; ++rt) {
-        p1[rt] = typename M::acc_t{0, 0, 0, 0};
-        p2[rt] = typename M::acc_t{0, 0, 0, 0};
-        p3[rt] = typename M::acc_t{0, 0, 0, 0};
-      }
-      if (g.small) {
-        R ur[4], ui[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          ur[e] = f[e * 64 + lane];
-          ui[e] = f[(4 + e) * 64 + lane];
-        }
-#pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-#pragma unroll
-          for (int gi = 0; gi < 2; ++gi) {
-            V xv = x[gi * 4 + ks];
-            // an empty asm keeps this path's operand sums from being merged with the
-            // other path's and hoisted above the branch (with every wait for the
-            // prefetched operands of a direct-in block)
-            asm volatile("" : "+v"(xv.x));
-            p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
-            p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
-            p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
-          }
-      } else {
-        R ur[16], ui[16];
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-          ur[e] = f[e * 64 + lane];
-          ui[e] = f[(16 + e) * 64 + lane];
-        }
-#pragma unroll
-        for (int ks = 0; ks < 8; ++ks) {
-          const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
-#pragma unroll
-          for (int rt = 0; rt < 2; ++rt) {
-            p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
-            p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
-            p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
-          }
-        }
-      }
+    };
+    // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
+    // P3 = (Ur + Ui)(Sr + Si); out = (P1 - P2) + i (P3 - P1 - P2). Accumulator
+    // pair index: row tile (5-bit block) or column group (4-bit block).
+    typename M::acc_t p1[2], p2[2], p3[2];
+    auto results = [&](int i) __attribute__((always_inline)) {
       if (DIN && first && i == NTL - 1) {
         // every tile's operands are consumed: the next unit's loads reuse their
         // registers (kept below the MFMAs, so no second register set is needed)
@@ -426,6 +383,72 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
         if (to_hbm) s[tb + mem(p)] = o;
         else lds[i][tpad(p)] = o;
       }
+    };
+    auto zero = [&]() __attribute__((always_inline)) {
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt) {
+        p1[rt] = typename M::acc_t{0, 0, 0, 0};
+        p2[rt] = typename M::acc_t{0, 0, 0, 0};
+        p3[rt] = typename M::acc_t{0, 0, 0, 0};
+      }
+    };
+    V x[8];
+    if (g.small) {
+      R ur[4], ui[4];
+      operands(0, x);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        ur[e] = f[e * 64 + lane];
+        ui[e] = f[(4 + e) * 64 + lane];
+      }
+      auto mfmas = [&]() __attribute__((always_inline)) {
+        zero();
+#pragma unroll
+        for (int ks = 0; ks < 4; ++ks)
+#pragma unroll
+          for (int gi = 0; gi < 2; ++gi) {
+            V xv = x[gi * 4 + ks];
+            // an empty asm keeps this path's operand sums from being merged with the
+            // other path's and hoisted above the branch (with every wait for the
+            // prefetched operands of a direct-in block)
+            asm volatile("" : "+v"(xv.x));
+            p1[gi] = M::mfma(ur[ks], xv.x, p1[gi]);
+            p2[gi] = M::mfma(ui[ks], xv.y, p2[gi]);
+            p3[gi] = M::mfma(ur[ks] + ui[ks], xv.x + xv.y, p3[gi]);
+          }
+      };
+      mfmas();
+      if (NTL == 2) {
+        // the second tile's operands go into the first tile's operand registers
+        // once its MFMAs are issued, and are read while those MFMAs run
+        __builtin_amdgcn_sched_barrier(0);
+        operands(NTL - 1, x);
+        results(0);
+        mfmas();
+        results(NTL - 1);
+      } else {
+        results(0);
+      }
+    } else {
+      R ur[16], ui[16];
+      operands(0, x);
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        ur[e] = f[e * 64 + lane];
+        ui[e] = f[(16 + e) * 64 + lane];
+      }
+      zero();
+#pragma unroll
+      for (int ks = 0; ks < 8; ++ks) {
+        const R br = x[ks].x, bi = x[ks].y, bs = x[ks].x + x[ks].y;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt) {
+          p1[rt] = M::mfma(ur[rt * 8 + ks], br, p1[rt]);
+          p2[rt] = M::mfma(ui[rt * 8 + ks], bi, p2[rt]);
+          p3[rt] = M::mfma(ur[rt * 8 + ks] + ui[rt * 8 + ks], bs, p3[rt]);
+        }
+      }
+      results(0);  // 32 x 32 blocks run one tile per step only
     }
     lds_barrier();
   };
