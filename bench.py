@@ -35,10 +35,10 @@ FP32_MFMA_PEAK_TFLOPS = 157.3
 # same counter (2 for FETCH_SIZE, 1 for WRITE_SIZE); summary in
 # profiles/block_kernels_brickwork28_pmc.md. Keyed by the amplitude size in bytes
 # (16 complex128, 8 complex64).
-TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2112924 + 4194306) * 1024),
-                        "fused": int((2 * 2101294 + 5148496) * 1024)},
-                   8: {"tblock": int((2 * 1049166 + 2097152) * 1024),
-                       "fused": int((2 * 1083983 + 2351668) * 1024)}}
+TRAFFIC28_BYTES = {16: {"tblock": int((2 * 2112806 + 4194304) * 1024),
+                        "fused": int((2 * 2101370 + 5145837) * 1024)},
+                   8: {"tblock": int((2 * 1049172 + 2097152) * 1024),
+                       "fused": int((2 * 1084233 + 2350587) * 1024)}}
 NBITS = 28
 DEPTH = 20
 SEED = 20
