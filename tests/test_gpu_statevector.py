@@ -318,8 +318,10 @@ def test_block5_random_circuits_vs_oracle(gpu, n, seed):
     assert rel_err(got, O.apply(n, psi, lower(els))) < TOL
 
 
-@pytest.mark.parametrize("n,depth", [(14, 6), (20, 4)])
+@pytest.mark.parametrize("n,depth", [(13, 6), (14, 6), (20, 4)])
 def test_block5_brickwork_vs_oracle(gpu, n, depth):
+    """Brickwork plans are mostly 16 x 16 blocks, so their passes take two tiles per
+    workgroup step; n = 13 has a single such step (two tiles) per pass."""
     els = brickwork(n, depth, 60 + n)
     psi = seeded_state(n, 60 + n)
     print(f"[case] block5 brickwork n={n} depth={depth}", flush=True)
