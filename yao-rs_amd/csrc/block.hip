@@ -1016,7 +1016,7 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits, b
       std::vector<size_t> left;
       for (size_t k : rem) {
         const uint64_t bm = bits_of(steps[k]);
-        if (nbits > kTT && !(bm & blocked) && popc(cur | bm) <= kTT && it.blocks.size() < size_t(kPassMax)) {
+        if (nbits >= kTT && !(bm & blocked) && popc(cur | bm) <= kTT && it.blocks.size() < size_t(kPassMax)) {
           cur |= bm;
           it.blocks.push_back(k);
         } else {
@@ -1024,7 +1024,7 @@ std::vector<PassItem> group_passes(const std::vector<BStep>& steps, int nbits, b
           left.push_back(k);
         }
       }
-      if (it.blocks.empty()) {  // n <= 12: one block per pass (k_block5)
+      if (it.blocks.empty()) {  // n < 12: one block per pass (k_block5)
         it.blocks.push_back(left.front());
         cur |= bits_of(steps[left.front()]);
         left.erase(left.begin());
@@ -1083,7 +1083,7 @@ int plan_passes(const std::vector<BStep>& steps, int nbits, std::vector<PassPlan
     // tile bits = the pass's bits padded with the lowest free bits to exactly 12
     uint64_t tb = it.bits;
     for (int b = 0; popc(tb) < kTT && b < nbits; ++b) tb |= uint64_t(1) << b;
-    if (popc(tb) != kTT || nbits <= kTT) return fail(YH_ERR_UNSUPPORTED, "block pass tile does not fit");
+    if (popc(tb) != kTT || nbits < kTT) return fail(YH_ERR_UNSUPPORTED, "block pass tile does not fit");
     if (it.blocks.size() > size_t(kPassMax)) return fail(YH_ERR_UNSUPPORTED, "block pass size out of range");
     pp.kind = 2;
     pp.tile = tb;
