@@ -56,11 +56,12 @@ def test_vqe_batch_every_configuration_vs_oracle(tile_bits, resident, threads):
     """Each fixed launch configuration against the oracle: instances above the tile
     size stream through HBM in 2 chunks, and at tile_bits 11 and 12 in 3 chunks
     (passes that visit the chunks back and forth), depth 0 included."""
-    inst = random_instances(16, seed=0, nmin=tile_bits - 2, nmax=min(20, tile_bits + 8), dmin=0, dmax=5)
+    inst = random_instances(16, seed=0, nmin=tile_bits - 2, nmax=min(21, tile_bits + 9), dmin=0, dmax=5)
     if tile_bits < 13:
-        assert inst.nq.max() >= 2 * tile_bits - 4  # a three-chunk instance is present
+        # memory bits 0..3 are in every tile and a chunk has tile_bits - 4 more
+        assert inst.nq.max() > 2 * tile_bits - 4  # a three-chunk instance is present
     want = oracle_energies(inst)
-    b = VqeBatch(32, 20, "c128").configure(tile_bits, 2, resident, threads)
+    b = VqeBatch(32, 21, "c128").configure(tile_bits, 2, resident, threads)
     got = b.energies(inst)
     assert b.config() == {"tile_bits": tile_bits, "blocks_per_cu": 2, "resident_grid": resident,
                           "step_threads": threads}

@@ -40,7 +40,11 @@ struct VqeInst {
 
 namespace {
 
-constexpr int kVqeLow = 5;
+// memory bits 0..3 are in every tile: runs of 16 amplitudes (256 B in complex128).
+// Four rather than five fixed bits leave one more bit per chunk, so instances up to
+// 20 qubits take two chunks at TB = 12 (three with five fixed bits): a third fewer
+// steps, each streaming the whole state.
+constexpr int kVqeLow = 4;
 
 template <int TB>
 __device__ __forceinline__ int vpad(int p) {
@@ -173,9 +177,9 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
 }  // namespace
 
 // Instances with n > TB: their states live in HBM (one buffer each) and every
-// instance advances one step per launch. The memory bits outside 0..4 form
-// k = npass chunks: chunk 0 is bits 5..TB-1, chunk c > 0 the c-th of k - 1 equal
-// parts of bits TB..n-1. The tile of a step holds bits 0..4 plus one chunk, so it
+// instance advances one step per launch. The memory bits outside 0..3 form
+// k = npass chunks: chunk 0 is bits 4..TB-1, chunk c > 0 the c-th of k - 1 equal
+// parts of bits TB..n-1. The tile of a step holds bits 0..3 plus one chunk, so it
 // has tbits <= TB bits and a work item (one workgroup) takes 2^(TB - tbits)
 // consecutive tiles: 2^TB amplitudes, and 2^(n - TB) items per instance in every
 // step. Step j holds chunk bounce(j): 0, 1, .., k-1, k-2, .., 0, 1, ..; with
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
 //     the chunk, the CZ chain (layer L is then complete), then layer L + 1 on every
 //     bit of the tile if L + 1 < depth;
 //   other j <= T: layer j / (k - 1) on the chunk;
-//   j >= T: energy terms: X on the chunk's bits, plus ZZ and X on bits 0..4 at j = T.
+//   j >= T: energy terms: X on the chunk's bits, plus ZZ and X on bits 0..3 at j = T.
 // Steps j <= T write the tile back; an instance takes T + k steps.
 struct VqeStep {
   int32_t inst;  // sorted instance index
@@ -247,7 +251,7 @@ __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big
   it.step = e.step;
   it.last = it.depth * (it.npass - 1);
   it.chunk = vqe_bounce(e.step, it.npass);
-  // tile positions 0..4 are memory bits 0..4, positions 5..tbits-1 the contiguous
+  // tile positions 0..3 are memory bits 0..3, positions 4..tbits-1 the contiguous
   // memory bits lo_bit..lo_bit+nh-1
   if (it.chunk == 0) {
     it.nh = TB - kVqeLow;
@@ -261,7 +265,7 @@ __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big
 }
 
 // memory index of LDS position p: tile (item << (TB - tbits)) + (p >> tbits) fills the
-// bits outside the tile in ascending order (zeros inserted at 0..4 and at the chunk)
+// bits outside the tile in ascending order (zeros inserted at 0..3 and at the chunk)
 template <int TB>
 __device__ __forceinline__ uint64_t vqe_mem_index(const VqeItem& it, int p) {
   const uint64_t t = (uint64_t(it.item) << (TB - it.tbits)) + uint64_t(p >> it.tbits);
@@ -375,7 +379,7 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
       }
     }
     if (j >= cur.last) {
-      // energy terms of the final state: ZZ (every amplitude) and X on bits 0..4 in
+      // energy terms of the final state: ZZ (every amplitude) and X on bits 0..3 in
       // step `last`, X on the chunk's bits in every energy step; one partial pair
       // per (chunk, item)
       const bool first_e = j == cur.last;
