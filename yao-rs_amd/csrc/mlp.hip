@@ -48,7 +48,12 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
                                                  const uint16_t* __restrict__ B, int ldb, int M, int N, int K,
                                                  int epi, const float* __restrict__ bias,
                                                  const float* __restrict__ zsrc, float* __restrict__ c32,
-                                                 uint16_t* __restrict__ c16, int ldc, int Mpad) {
+                                                 uint16_t* __restrict__ c16, int ldc, int Mpad, int kc,
+                                                 size_t zstride) {
+  // split K: workgroup z sums k in [z kc, (z + 1) kc) and stores (EPI_STORE only) to
+  // c32 + z zstride; kc is a multiple of 16
+  const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+  c32 += blockIdx.z * zstride;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
   const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
   for (int i = 0; i < 2; ++i) ap[i] = A + size_t(min(m0 + 16 * i + r, M - 1)) * lda + kq;
 #pragma unroll
   for (int j = 0; j < 2; ++j) bp[j] = B + size_t(min(n0 + 16 * j + r, N - 1)) * ldb + kq;
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int k0 = kb; k0 < ke; k0 += 16) {
     s4 a[2], b[2];
 #pragma unroll
     for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const s4*>(ap[i] + k0);
@@ -175,6 +180,16 @@ __global__ __launch_bounds__(256) void k_out_bwd_dz(const float* __restrict__ do
   dz16[e] = f2bf(g);
 }
 
+// out[i] = sum over z = 0..S-1, in that order, of part[z stride + i]
+__global__ __launch_bounds__(256) void k_sum_splits(const float* __restrict__ part, int S, size_t stride,
+                                                    float* __restrict__ out, size_t n) {
+  const size_t i = size_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  float s = part[i];
+  for (int z = 1; z < S; ++z) s += part[z * stride + i];
+  out[i] = s;
+}
+
 // column sums of a row-major fp32 [M x N] matrix (one wave per column)
 __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ x, int M, int N, float* __restrict__ out) {
   const int c = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -264,6 +279,8 @@ __global__ __launch_bounds__(256) void k_weights_bf16(const float* __restrict__ 
 
 inline unsigned blocks(size_t n) { return unsigned((n + 255) / 256); }
 
+constexpr int kMaxSplit = 16;  // K-splits of a weight-gradient GEMM
+
 }  // namespace
 
 }  // namespace yh
@@ -278,6 +295,7 @@ struct yh_mlp {
   DevBuf params, grads, am, av;
   std::vector<DevBuf*> wbf, wtbf, zbuf, hbf, dz32, dz16, tmp32;
   DevBuf x32, xbf, hT, dzT, out, dout, y, lossp, eacc, idx, data_x, data_y;
+  DevBuf wsplit;  // split-K partial weight gradients, kMaxSplit x width x max fan-in
   size_t data_rows = 0;
   int64_t adam_t = 0;
   ~yh_mlp() {
@@ -291,6 +309,9 @@ struct yh_mlp {
 namespace yh {
 
 static float* fptr(DevBuf& b, size_t off = 0) { return static_cast<float*>(b.p) + off; }
+
+// K-splits of a weight-gradient GEMM over Mp rows: chunks of at least 64 rows
+static int split_count(int Mp) { return std::max(1, std::min(kMaxSplit, Mp / 64)); }
 static uint16_t* hptr(DevBuf& b) { return static_cast<uint16_t*>(b.p); }
 
 // forward over `rows` rows already in m->xbf (bf16 [rows x feat]); output in m->out
@@ -303,7 +324,7 @@ static int mlp_forward(yh_mlp* m, int rows) {
     dim3 grid(N / 64, Mp / 64);
     hipLaunchKernelGGL(k_gemm_nt, grid, dim3(256), 0, st, hin, K, hptr(*m->wbf[l]), K, rows, N, K, int(EPI_BIAS_GELU),
                        fptr(m->params, m->boff[l]), static_cast<const float*>(nullptr), fptr(*m->zbuf[l]),
-                       hptr(*m->hbf[l]), N, Mp);
+                       hptr(*m->hbf[l]), N, Mp, K, size_t(0));
     hin = hptr(*m->hbf[l]);
   }
   const int L = m->hidden;
@@ -334,14 +355,22 @@ static int mlp_backward(yh_mlp* m, int rows) {
     hipLaunchKernelGGL(k_transpose, dim3((N + 31) / 32, (Mp + 31) / 32), dim3(256), 0, st, hptr(*m->dz16[l]), Mp, N,
                        hptr(m->dzT));
     hipLaunchKernelGGL(k_transpose, dim3((K + 31) / 32, (Mp + 31) / 32), dim3(256), 0, st, hprev, Mp, K, hptr(m->hT));
-    hipLaunchKernelGGL(k_gemm_nt, dim3((K + 63) / 64, N / 64), dim3(256), 0, st, hptr(m->dzT), Mp, hptr(m->hT), Mp, N,
-                       K, Mp, int(EPI_STORE), static_cast<const float*>(nullptr), static_cast<const float*>(nullptr),
-                       fptr(m->grads, m->woff[l]), static_cast<uint16_t*>(nullptr), K, N);
+    // the reduction runs over the Mp batch rows: split it so that the launch has
+    // (K / 64) x (N / 64) x S workgroups, then sum the S partials in a fixed order
+    const int S = split_count(Mp), kc = (Mp / S + 15) / 16 * 16, nz = (Mp + kc - 1) / kc;
+    float* dw = nz > 1 ? fptr(m->wsplit) : fptr(m->grads, m->woff[l]);
+    hipLaunchKernelGGL(k_gemm_nt, dim3((K + 63) / 64, N / 64, nz), dim3(256), 0, st, hptr(m->dzT), Mp, hptr(m->hT), Mp,
+                       N, K, Mp, int(EPI_STORE), static_cast<const float*>(nullptr),
+                       static_cast<const float*>(nullptr), dw, static_cast<uint16_t*>(nullptr), K, N, kc,
+                       size_t(N) * K);
+    if (nz > 1)
+      hipLaunchKernelGGL(k_sum_splits, dim3(blocks(size_t(N) * K)), dim3(256), 0, st, dw, nz, size_t(N) * K,
+                         fptr(m->grads, m->woff[l]), size_t(N) * K);
     if (l > 0) {
       // dz_{l-1} = (dz_l W_l) * gelu'(z_{l-1})
       hipLaunchKernelGGL(k_gemm_nt, dim3((K + 63) / 64, Mp / 64), dim3(256), 0, st, hptr(*m->dz16[l]), N,
                          hptr(*m->wtbf[l]), N, rows, K, N, int(EPI_GELU_GRAD), static_cast<const float*>(nullptr),
-                         fptr(*m->zbuf[l - 1]), fptr(*m->dz32[l - 1]), hptr(*m->dz16[l - 1]), K, Mp);
+                         fptr(*m->zbuf[l - 1]), fptr(*m->dz32[l - 1]), hptr(*m->dz16[l - 1]), K, Mp, N, size_t(0));
     }
   }
   return YH_OK;
@@ -378,6 +407,7 @@ static int mlp_alloc(yh_mlp* m) {
   YH_TRY(m->xbf.ensure(B * m->feat * 2));
   YH_TRY(m->hT.ensure(B * maxk * 2));
   YH_TRY(m->dzT.ensure(B * W * 2));
+  YH_TRY(m->wsplit.ensure(size_t(kMaxSplit) * W * maxk * 4));
   YH_TRY(m->out.ensure(B * 4));
   YH_TRY(m->dout.ensure(B * 4));
   YH_TRY(m->y.ensure(B * 4));
