@@ -421,16 +421,20 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
                     "tflops_algorithmic": round(3 * fwd_flops * train_pairs * epochs / tt / 1e12, 2)}
     log(f"surrogate: trained {epochs} epochs in {tt:.2f} s, losses {losses}")
     cand = s.sample_angles(candidates, seed=202)
-    s.screen(cand[:4096], keep_frac=keep_frac)  # warm the c128 simulator and the predictor
+    # warm-up on another pool of the same size: the simulator autotunes and captures
+    # its graph once per instance-shape list (here: as many instances as are kept),
+    # a one-off per ansatz like training; that first call is reported on its own
+    _, _, _, tm0 = s.screen(s.sample_angles(candidates, seed=201), keep_frac=keep_frac, dtype="c128")
     idx, exact, pred, tm = s.screen(cand, keep_frac=keep_frac, dtype="c128")
     t_screen = tm["predict_s"] + tm["reeval_s"]
     res["screen"] = {"predict_s": round(tm["predict_s"], 4), "reeval_s": round(tm["reeval_s"], 4),
-                     "reevaluated": int(idx.size),
+                     "reevaluated": int(idx.size), "first_call_reeval_s": round(tm0["reeval_s"], 4),
                      "predictions_per_s": round(candidates / tm["predict_s"], 1),
                      "screened_candidates_per_s": round(candidates / t_screen, 1)}
     # the surrogate's error, judged by the exact simulator on a sample of the candidates
     rng = np.random.default_rng(303)
     samp = rng.choice(candidates, size=mae_sample, replace=False)
+    s.exact_energies(s.sample_angles(mae_sample, seed=304), dtype="c128")  # same shape list: autotune, graph
     t0 = time.perf_counter()
     ex = s.exact_energies(cand[samp], dtype="c128")
     tb = time.perf_counter() - t0
@@ -453,7 +457,9 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
     res["screened_gain_vs_plain"] = round(candidates / t_screen / res["plain_c128_evals_per_s"], 1)
     res["screened_gain_vs_batched_exact"] = round(candidates / t_screen / res["batched_c128_evals_per_s"], 2)
     res["note"] = ("screened candidates/s count surrogate predictions plus exact re-evaluation of the kept "
-                   "fraction; data generation and training are reported separately (one-off per ansatz)")
+                   "fraction, with the simulator already tuned for the kept count; data generation, training "
+                   "and the simulator's first call (autotune + graph capture, first_call_reeval_s) are reported "
+                   "separately (one-off per ansatz)")
     log(f"surrogate: {res}")
     return res
 
