@@ -9,7 +9,10 @@ on a bounded sample of the same workload on this host. The same circuit is also
 timed gate by gate and through LDS tile passes, and the other workloads (complex64,
 batched VQE, noisy density matrix, learned surrogate) are reported beside it.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--skip-extra]
+       python bench.py --profile WORKLOAD [--nbits N] [--depth D]
+The --profile form runs one named workload once (for rocprofv3 kernel traces and
+counter runs) and prints progress lines instead of the JSON line; see PROFILE_WORKLOADS.
 """
 import argparse
 import ctypes
@@ -23,7 +26,6 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "yao-rs_amd"))
-sys.path.insert(0, os.path.join(ROOT, "tests"))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
@@ -95,7 +97,11 @@ def main():
     ap.add_argument("--skip-extra", action="store_true", help="only the 28-qubit complex128 brickwork")
     ap.add_argument("--nbits", type=int, default=NBITS)
     ap.add_argument("--depth", type=int, default=DEPTH)
+    ap.add_argument("--profile", choices=sorted(PROFILE_WORKLOADS), help="run one workload for a profiler")
     args = ap.parse_args()
+    if args.profile:
+        profile_workload(args.profile, args.nbits, args.depth)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this backend runs on one GPU; multi-GPU sharding is design only "
                                    "(see DESIGN.md), run with --gpus 1"}))
@@ -104,7 +110,7 @@ def main():
     import yaohip
     from yaohip import _abi as A
     from yaohip.circuit import Circuit
-    from fixture_circuits import brickwork
+    from yaohip.workloads import brickwork
 
     n, depth = args.nbits, args.depth
     els = brickwork(n, depth, SEED)
@@ -299,7 +305,7 @@ def bench_qft12(log, reps=200):
     runs the circuit in one launch of one workgroup."""
     import yaohip
     from yaohip.circuit import Circuit, OperatorPolynomial, OperatorString, Op
-    from fixture_circuits import qft_with_swaps
+    from yaohip.workloads import qft_with_swaps
     n = 12
     rng = np.random.default_rng(2024)
     psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
@@ -474,7 +480,7 @@ def bench_dm(log, n=14, depth=10):
     import yaohip
     from yaohip import _abi as A
     from yaohip.circuit import Circuit
-    from fixture_circuits import noisy_brickwork
+    from yaohip.workloads import noisy_brickwork
     els = noisy_brickwork(n, depth, 3)
     circ = Circuit.qubits(n, els)
     ngates = sum(1 for e in els if hasattr(e, "gate"))
@@ -526,6 +532,79 @@ def gpu_zz(dm, i):
     import yaohip
     from yaohip.circuit import OperatorPolynomial, OperatorString, Op
     return yaohip.expect_dm(dm, OperatorPolynomial([1.0], [OperatorString([(i, Op.Z), (i + 1, Op.Z)])])).real
+
+
+PROFILE_WORKLOADS = {
+    "block": "random brickwork (--nbits, --depth) as fused MFMA blocks, complex128, one application",
+    "block64": "the same circuit as fused MFMA blocks, complex64",
+    "tiled": "the same circuit through LDS tile passes (max_fuse_qubits 4)",
+    "gbg": "the same circuit gate by gate",
+    "copy": "three device-to-device copies of a 2^nbits complex128 state (the reference for scaling "
+            "FETCH_SIZE / WRITE_SIZE counters)",
+    "k1q_bits": "Rx on every qubit of a 2^nbits complex128 state, k_1q launch time per target bit",
+    "vqe": "one timed batch of 4096 ansatz instances (n 10..20, depth 4..8), complex64, after the autotuned first",
+    "vqe128": "the same batch in complex128",
+    "dm": "noisy brickwork on a 2^nbits x 2^nbits density matrix, per-gate fused passes",
+    "dmblk": "the same on the superoperator block path",
+    "mlp_train": "surrogate training: 2 epochs over 65536 random pairs, 288 angles, batch 4096",
+}
+
+
+def profile_workload(w, n, depth):
+    """One named workload, printing a line that names it before it touches the GPU."""
+    print(f"[prof] workload={w} nbits={n} depth={depth}", flush=True)
+    import yaohip
+    from yaohip import _abi as A
+    from yaohip.circuit import Circuit
+    from yaohip.workloads import brickwork, noisy_brickwork
+    if w in ("tiled", "gbg", "block", "block64"):
+        reg = yaohip.ArrayReg.zero_state(n, "c64" if w == "block64" else "c128")
+        circ = Circuit.qubits(n, brickwork(n, depth, SEED))
+        t0 = time.perf_counter()
+        yaohip.apply_inplace(circ, reg, {"tiled": 4, "gbg": 0, "block": 5, "block64": 5}[w])
+        print(f"[prof] {w} {time.perf_counter() - t0:.3f} s norm {reg.norm():.15f}", flush=True)
+    elif w == "copy":
+        a = yaohip.ArrayReg.zero_state(n)
+        b = yaohip.ArrayReg.zero_state(n)
+        for _ in range(3):
+            A.check(A.lib().yh_reg_copy(b._h, a._h))
+        print(f"[prof] copy of {16 << n} bytes x3 done", flush=True)
+    elif w in ("vqe", "vqe128"):
+        from yaohip.vqe import VqeBatch, random_instances
+        inst = random_instances(4096, seed=4)
+        b = VqeBatch(4096, 20, "c64" if w == "vqe" else "c128")
+        b.energies(inst)
+        t0 = time.perf_counter()
+        e = b.energies(inst)
+        print(f"[prof] vqe batch {time.perf_counter() - t0:.4f} s mean E {e.mean():.6f}", flush=True)
+    elif w == "k1q_bits":
+        from yaohip.circuit import Gate, put
+        reg = yaohip.ArrayReg.zero_state(n)
+        L = A.lib()
+        for loc in range(n):
+            circ = Circuit.qubits(n, [put([loc], Gate.Rx(0.3))] * 4)
+            yaohip.apply_inplace(circ, reg)
+            A.check(L.yh_reg_timing_enable(reg._h, 1))
+            yaohip.apply_inplace(circ, reg)
+            c, ms = ctypes.c_uint64(), ctypes.c_double()
+            A.check(L.yh_reg_timing_read(reg._h, A.K_1Q, ctypes.byref(c), ctypes.byref(ms)))
+            A.check(L.yh_reg_timing_enable(reg._h, 0))
+            us = 1e3 * ms.value / c.value
+            print(f"[prof] k_1q target bit {n - 1 - loc:2d}: {us:8.1f} us, "
+                  f"{2 * 16 * (1 << n) / (us * 1e-6) / 1e12:.2f} TB/s", flush=True)
+    elif w in ("dm", "dmblk"):
+        dm = yaohip.DensityMatrix.zero_state(n)
+        dm.apply(Circuit.qubits(n, noisy_brickwork(n, depth, 3)), fused=True if w == "dm" else "blocks")
+        print(f"[prof] {w} trace {dm.trace().real:.15f}", flush=True)
+    elif w == "mlp_train":
+        from yaohip.surrogate import Mlp
+        rng = np.random.default_rng(5)
+        x = rng.uniform(-0.8, 0.8, size=(65536, 288)).astype(np.float32)
+        y = np.cos(x[:, :8]).sum(axis=1).astype(np.float32)
+        m = Mlp(288, width=256, hidden=4, max_batch=4096, seed=1, encoding="trig")
+        t0 = time.perf_counter()
+        losses = m.train(x, y, batch=4096, epochs=2, lr=1e-3, seed=3)
+        print(f"[prof] mlp_train 2 epochs {time.perf_counter() - t0:.3f} s losses {losses}", flush=True)
 
 
 if __name__ == "__main__":

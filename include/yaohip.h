@@ -115,7 +115,7 @@ int yh_device_name(char *buf, size_t len);
 int yh_copy_bandwidth(size_t bytes, int reps, double *gbps);
 
 /* ---- ArrayReg (src/register.rs) ---- */
-int yh_reg_create(int nbits, int dtype, yh_reg **out);            /* zero_state :21-26  */
+int yh_reg_create(int nbits, int dtype, yh_reg **out);            /* zero_state :21-26; nbits <= 35 */
 int yh_reg_destroy(yh_reg *reg);
 int yh_reg_nbits(const yh_reg *reg, int *nbits);                  /* nqubits :87-89     */
 int yh_reg_dtype(const yh_reg *reg, int *dtype);
@@ -128,6 +128,18 @@ int yh_reg_normalize(yh_reg *reg);                                /* normalize :
 int yh_reg_fidelity(const yh_reg *a, const yh_reg *b, double *out); /* fidelity :115-124 */
 /* Device address of the amplitudes (interleaved re,im; double or float by dtype). */
 int yh_reg_device_ptr(const yh_reg *reg, void **ptr);
+/* Constructors (src/register.rs), each a new register filled on the device: */
+/* product_state (:28-33): the one-hot state of basis index `index`, i.e. BitStr::onehot
+ * (bitbasis/src/bitstr.rs:24-29); index < 2^nbits. */
+int yh_reg_product_state(int nbits, int dtype, uint64_t index, yh_reg **out);
+int yh_reg_uniform_state(int nbits, int dtype, yh_reg **out);       /* uniform_state :47-54 */
+int yh_reg_ghz_state(int nbits, int dtype, yh_reg **out);           /* ghz_state :56-63     */
+/* deterministic_state (:71-84): amplitude k = (cos 0.1k, sin 0.2k), normalised. */
+int yh_reg_deterministic_state(int nbits, int dtype, yh_reg **out);
+/* rand_state (:35-45) with the caller's stream: amplitude k = (u[2k] - 0.5,
+ * u[2k+1] - 0.5), normalised; n_uniforms = 2 * 2^nbits f64 uniforms, the values
+ * `rng.random::<f64>()` returns in the reference's loop order. */
+int yh_reg_rand_state(int nbits, int dtype, const double *uniforms, size_t n_uniforms, yh_reg **out);
 
 /* ---- state files: yao-state-v1 (yao-cli/src/state_io.rs) ---- */
 /* write_state (state_io.rs:18-47): header line + little-endian (re, im) f64 pairs,
@@ -228,12 +240,23 @@ int yh_measure(yh_reg *reg, const int32_t *locs, int nlocs, int post, uint64_t r
 /* ---- DensityMatrix (src/density_matrix.rs) ---- */
 int yh_dm_create(int nbits, yh_dm **out);                          /* zero_state :19-21 */
 int yh_dm_from_reg(const yh_reg *reg, yh_dm **out);                /* from_reg :23-35   */
+/* mixed (density_matrix.rs:37-61): sum_j weights[j] |psi_j><psi_j| over nregs
+ * complex128 registers of one size, accumulated per entry in register order. */
+int yh_dm_mixed(const double *weights, const yh_reg *const *regs, size_t nregs, yh_dm **out);
+/* density_matrix_from_reg (density_matrix.rs:266-272): the reduced density matrix
+ * on the kept `locs` (read as bit positions, like partial_tr; the others are traced
+ * out), computed from the amplitudes without forming the full 4^n matrix; entries
+ * are summed in the reference's order. */
+int yh_dm_from_reg_locs(const yh_reg *reg, const int32_t *locs, int nlocs, yh_dm **out);
 int yh_dm_destroy(yh_dm *dm);
 int yh_dm_nbits(const yh_dm *dm, int *nbits);
 int yh_dm_upload(yh_dm *dm, const double *rho, size_t n_entries);
 int yh_dm_download(const yh_dm *dm, double *rho, size_t n_entries);
 /* Register::apply for DensityMatrix (:244-259): gate -> U rho U^dagger (:161-178),
- * channel -> sum_i K_i rho K_i^dagger (:181-224). One pass over rho per element. */
+ * channel -> sum_i K_i rho K_i^dagger (:181-224). One pass over rho per element on at
+ * most 2 qubits; a channel on more qubits (or with more than 64 Kraus operators)
+ * sums its Kraus branches as the reference does (a branch copy of rho per operator,
+ * K on the row bits, conj(K) on the column bits, accumulated in order). */
 int yh_dm_apply(yh_dm *dm, int circuit_nbits, const yh_op *ops, size_t nops);
 /* Same result; each gate is fused with the 1-qubit channels that follow it on its
  * target qubits into one superoperator pass. */
@@ -253,9 +276,10 @@ int yh_dm_purity(const yh_dm *dm, double *out);                    /* purity :69
  * reference's loc here); kept bit idx of the reduced index is the idx-th kept bit.
  * Entries are summed in the reference's order (bit-identical). */
 int yh_dm_partial_tr(const yh_dm *dm, const int32_t *traced, int ntraced, yh_dm **out);
-/* von_neumann_entropy (density_matrix.rs:113-122), nbits <= 10: the matrix is
- * downloaded and its Hermitian eigenvalues computed on the host (Householder
- * tridiagonalisation + implicit QL), as the reference does with faer. */
+/* von_neumann_entropy (density_matrix.rs:113-122). The Hermitian eigenvalues are
+ * computed on the host (Householder tridiagonalisation + implicit QL) for
+ * nbits <= 10, else on the device with rocSOLVER's zheevd on a copy of rho (the
+ * reference uses faer's eigensolver on the CPU). */
 int yh_dm_entropy(const yh_dm *dm, double *out);
 /* expect_dm (expect.rs:38-72): Pauli sites are LSB-first bit positions. */
 int yh_dm_expect(const yh_dm *dm, size_t nterms, const double *coeffs,

@@ -20,6 +20,12 @@
 //   itercontrol           : bitbasis/src/iterate_control.rs:166-219 index lists;
 //   sampling              : cumsum boundary cases (r exactly at a partial sum);
 //   collapse/reset/remove : closed forms on GHZ / product states.
+//
+// Threads: orc_set_threads(k) lets the loops over independent amplitude groups of
+// the instructs (and the density matrix's branch accumulation and outer product)
+// run on k OpenMP threads. Each group is updated by the same arithmetic in the same
+// order, so results do not depend on k; the default is one thread (the
+// reference is single-threaded), and the CPU baseline is timed that way.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -47,6 +53,11 @@ inline C divr(C a, double d) { return C{a.re / d, a.im / d}; }
 
 using Vec = std::vector<C>;
 using usize = uint64_t;
+
+int g_threads = 1;                        // orc_set_threads
+constexpr usize kParMin = usize(1) << 14;  // smaller loops stay on one thread
+#define ORC_PAR_REGION(n) if (g_threads > 1 && (n) >= kParMin) num_threads(g_threads)
+#define ORC_PAR(n) ORC_PAR_REGION(n) schedule(static)
 
 // ---------------- bitbasis (bitbasis/src/bit_ops.rs, iterate_control.rs) ----------------
 inline usize indicator(usize k) { return usize(1) << k; }
@@ -130,11 +141,10 @@ void for_each_single_control_base(usize total, usize target_bit, usize ctrl_bit,
     usize middle_iters = usize(1) << (ctrl_bit - target_bit - 1);
     usize outer_step = usize(1) << (ctrl_bit + 1);
     usize ctrl_offset = ctrl_val << ctrl_bit;
+#pragma omp parallel for collapse(3) ORC_PAR(total)
     for (usize outer = 0; outer < total; outer += outer_step) {
-      usize ctrl_base = outer + ctrl_offset;
       for (usize m = 0; m < middle_iters; ++m) {
-        usize mb = ctrl_base + (m << (target_bit + 1));
-        for (usize low = 0; low < low_span; ++low) f(mb + low);
+        for (usize low = 0; low < low_span; ++low) f(outer + ctrl_offset + (m << (target_bit + 1)) + low);
       }
     }
   } else {
@@ -142,10 +152,10 @@ void for_each_single_control_base(usize total, usize target_bit, usize ctrl_bit,
     usize middle_iters = usize(1) << (target_bit - ctrl_bit - 1);
     usize outer_step = usize(1) << (target_bit + 1);
     usize ctrl_offset = ctrl_val << ctrl_bit;
+#pragma omp parallel for collapse(3) ORC_PAR(total)
     for (usize outer = 0; outer < total; outer += outer_step) {
       for (usize m = 0; m < middle_iters; ++m) {
-        usize mb = outer + (m << (ctrl_bit + 1)) + ctrl_offset;
-        for (usize low = 0; low < low_span; ++low) f(mb + low);
+        for (usize low = 0; low < low_span; ++low) f(outer + (m << (ctrl_bit + 1)) + ctrl_offset + low);
       }
     }
   }
@@ -154,6 +164,7 @@ void for_each_single_control_base(usize total, usize target_bit, usize ctrl_bit,
 void instruct_x(C* s, usize nbits, usize loc) {  // :82-92
   usize mask = indicator(loc_to_bit(nbits, loc));
   usize dim = usize(1) << nbits;
+#pragma omp parallel for ORC_PAR(dim)
   for (usize b = 0; b < dim; ++b)
     if ((b & mask) == 0) std::swap(s[b], s[b ^ mask]);
 }
@@ -175,6 +186,7 @@ void instruct_x_controlled(C* s, usize nbits, usize loc, const std::vector<usize
     return;
   }
   Controller ctrl = controller(to_bits(nbits, cl), cb);
+#pragma omp parallel for ORC_PAR(dim)
   for (usize b = 0; b < dim; ++b)
     if ((b & mask) == 0 && ctrl(b)) std::swap(s[b], s[b ^ mask]);
 }
@@ -183,18 +195,20 @@ void instruct_1q(C* s, usize len, usize loc, C a, C b, C c, C d) {  // :138-158
   usize nbits = __builtin_ctzll(len);
   usize bit = loc_to_bit(nbits, loc);
   usize step1 = usize(1) << bit, step2 = usize(1) << (bit + 1);
+#pragma omp parallel for collapse(2) ORC_PAR(len)
   for (usize j = 0; j < len; j += step2)
-    for (usize i = j; i < j + step1; ++i) u1rows(s, i, i + step1, a, b, c, d);
+    for (usize t = 0; t < step1; ++t) u1rows(s, j + t, j + t + step1, a, b, c, d);
 }
 
 void instruct_1q_diag(C* s, usize len, usize loc, C d0, C d1) {  // :170-184
   usize nbits = __builtin_ctzll(len);
   usize bit = loc_to_bit(nbits, loc);
   usize step1 = usize(1) << bit, step2 = usize(1) << (bit + 1);
+#pragma omp parallel for collapse(2) ORC_PAR(len)
   for (usize j = 0; j < len; j += step2)
-    for (usize i = j; i < j + step1; ++i) {
-      s[i] *= d0;
-      s[i + step1] *= d1;
+    for (usize t = 0; t < step1; ++t) {
+      s[j + t] *= d0;
+      s[j + t + step1] *= d1;
     }
 }
 
@@ -239,6 +253,7 @@ void instruct_2q(C* s, usize nbits, const std::vector<usize>& locs, const C* gat
   sort_unitary_2q(gate, loc_to_bit(nbits, locs[0]), loc_to_bit(nbits, locs[1]), g, sb);
   std::vector<usize> raw = locs_raw_2(nbits, sb);
   IterControl ic = itercontrol(nbits, {sb[0], sb[1]}, {0, 0});
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     usize inds[4] = {raw[0] + base, raw[1] + base, raw[2] + base, raw[3] + base};
@@ -258,6 +273,7 @@ void instruct_2q_diag(C* s, usize nbits, const std::vector<usize>& locs, const C
   }
   std::vector<usize> raw = locs_raw_2(nbits, sb);
   IterControl ic = itercontrol(nbits, {sb[0], sb[1]}, {0, 0});
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     for (int q = 0; q < 4; ++q) s[raw[q] + base] *= diag[q];
@@ -267,6 +283,7 @@ void instruct_2q_diag(C* s, usize nbits, const std::vector<usize>& locs, const C
 void instruct_swap(C* s, usize nbits, const std::vector<usize>& locs) {  // :305-320
   usize m0 = indicator(loc_to_bit(nbits, locs[0])), m1 = indicator(loc_to_bit(nbits, locs[1]));
   usize sm = m0 | m1, dim = usize(1) << nbits;
+#pragma omp parallel for ORC_PAR(dim)
   for (usize b = 0; b < dim; ++b)
     if ((b & m0) == 0 && (b & m1) == m1) std::swap(s[b], s[b ^ sm]);
 }
@@ -285,15 +302,19 @@ void instruct_nq(C* s, usize nbits, const std::vector<usize>& locs, const C* gat
     for (usize idx = 0; idx < nl; ++idx) acc |= ((cfg >> idx) & 1) << tb[idx];
     raw[cfg] = acc;
   }
-  std::vector<C> temp(gd);
   IterControl ic = itercontrol(nbits, locked, vals);
-  for (usize k = 0; k < ic.n; ++k) {
-    usize base = ic.get(k);
-    for (usize row = 0; row < gd; ++row) {
-      temp[row] = cx(0, 0);
-      for (usize col = 0; col < gd; ++col) temp[row] += gate[row * gd + col] * s[base + raw[col]];
+#pragma omp parallel ORC_PAR_REGION(ic.n)
+  {
+    std::vector<C> temp(gd);
+#pragma omp for schedule(static)
+    for (usize k = 0; k < ic.n; ++k) {
+      usize base = ic.get(k);
+      for (usize row = 0; row < gd; ++row) {
+        temp[row] = cx(0, 0);
+        for (usize col = 0; col < gd; ++col) temp[row] += gate[row * gd + col] * s[base + raw[col]];
+      }
+      for (usize r = 0; r < gd; ++r) s[base + raw[r]] = temp[r];
     }
-    for (usize r = 0; r < gd; ++r) s[base + raw[r]] = temp[r];
   }
 }
 
@@ -311,6 +332,7 @@ void instruct_1q_controlled(C* s, usize nbits, usize loc, C a, C b, C c, C d,
   std::vector<usize> vals = cb;
   vals.push_back(0);
   IterControl ic = itercontrol(nbits, locked, vals);
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     u1rows(s, base, base + step, a, b, c, d);
@@ -331,6 +353,7 @@ void instruct_2q_controlled(C* s, usize nbits, const std::vector<usize>& locs, c
   vals.push_back(0);
   vals.push_back(0);
   IterControl ic = itercontrol(nbits, locked, vals);
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     usize inds[4] = {raw[0] + base, raw[1] + base, raw[2] + base, raw[3] + base};
@@ -354,6 +377,7 @@ void instruct_1q_diag_controlled(C* s, usize nbits, usize loc, C d0, C d1,
   std::vector<usize> vals = cb;
   vals.push_back(0);
   IterControl ic = itercontrol(nbits, locked, vals);
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     s[base] *= d0;
@@ -379,6 +403,7 @@ void instruct_2q_diag_controlled(C* s, usize nbits, const std::vector<usize>& lo
   vals.push_back(0);
   vals.push_back(0);
   IterControl ic = itercontrol(nbits, locked, vals);
+#pragma omp parallel for ORC_PAR(ic.n)
   for (usize k = 0; k < ic.n; ++k) {
     usize base = ic.get(k);
     for (int q = 0; q < 4; ++q) s[raw[q] + base] *= diag[q];
@@ -948,7 +973,9 @@ int dm_apply_channel(usize n, Vec& rho, const yh_op& op) {  // :181-224
     yh_op gc = custom_gate(Kc, d, dg, pos, s2);
     for (int k = 0; k < gc.nt; ++k) gc.targets[k] += int(n);
     dispatch(2 * n, branch.data(), gc);
-    for (size_t i = 0; i < acc.size(); ++i) acc[i] += branch[i];
+    const usize len = acc.size();
+#pragma omp parallel for ORC_PAR(len)
+    for (usize i = 0; i < len; ++i) acc[i] += branch[i];
   }
   rho = acc;
   return 0;
@@ -1131,6 +1158,13 @@ int orc_apply(int nbits, int circuit_nbits, double* state, const yh_op* ops, siz
   for (size_t e = 0; e < nops; ++e)
     if (ops[e].elem == YH_ELEM_GATE) dispatch(nbits, asC(state), ops[e]);
   return 0;
+}
+
+// Threads for the parallel loops (k >= 1); returns the previous value.
+int orc_set_threads(int k) {
+  const int prev = g_threads;
+  g_threads = k < 1 ? 1 : k;
+  return prev;
 }
 
 int orc_dispatch(int nbits, double* state, const yh_op* op) {
@@ -1318,10 +1352,53 @@ int orc_superop(const yh_op* ch, double* out, int* d2out) {
 }
 
 // DensityMatrix::from_reg (density_matrix.rs:23-35)
+// deterministic_state (register.rs:71-84): (cos 0.1k, sin 0.2k), divided by the
+// square root of the sequential sum of norm_sqr
+int orc_deterministic_state(int nbits, double* out) {
+  const usize dim = usize(1) << nbits;
+  C* s = asC(out);
+  for (usize k = 0; k < dim; ++k) {
+    const double kf = double(k);
+    s[k] = cx(std::cos(0.1 * kf), std::sin(0.2 * kf));
+  }
+  double acc = 0.0;
+  for (usize k = 0; k < dim; ++k) acc += norm_sqr(s[k]);
+  const double nrm = std::sqrt(acc);
+  for (usize k = 0; k < dim; ++k) s[k] = divr(s[k], nrm);
+  return 0;
+}
+
+// rand_state (register.rs:35-45) with the uniforms of the caller's stream in the
+// reference's draw order: re then im per amplitude
+int orc_rand_state(int nbits, const double* u, double* out) {
+  const usize dim = usize(1) << nbits;
+  C* s = asC(out);
+  for (usize k = 0; k < dim; ++k) s[k] = cx(u[2 * k] - 0.5, u[2 * k + 1] - 0.5);
+  double acc = 0.0;
+  for (usize k = 0; k < dim; ++k) acc += norm_sqr(s[k]);
+  const double nrm = std::sqrt(acc);
+  for (usize k = 0; k < dim; ++k) s[k] = divr(s[k], nrm);
+  return 0;
+}
+
+// mixed (density_matrix.rs:37-61): states holds nregs registers back to back
+int orc_dm_mixed(int nbits, int nregs, const double* weights, const double* states, double* rho) {
+  const usize dim = usize(1) << nbits;
+  C* r = asC(rho);
+  for (usize e = 0; e < dim * dim; ++e) r[e] = cx(0, 0);
+  for (int j = 0; j < nregs; ++j) {
+    const C* psi = asC(states) + usize(j) * dim;
+    for (usize row = 0; row < dim; ++row)
+      for (usize col = 0; col < dim; ++col) r[row * dim + col] += cx(weights[j], 0.0) * psi[row] * conj(psi[col]);
+  }
+  return 0;
+}
+
 int orc_dm_from_reg(int nbits, const double* state, double* rho) {
   usize dim = usize(1) << nbits;
   const C* s = asC(state);
   C* r = asC(rho);
+#pragma omp parallel for ORC_PAR(dim * dim)
   for (usize row = 0; row < dim; ++row)
     for (usize col = 0; col < dim; ++col) r[row * dim + col] = s[row] * conj(s[col]);
   return 0;

@@ -36,6 +36,7 @@ def lib():
                                             P(i32), P(i32), ctypes.c_int]),
             "orc_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), vp, ctypes.c_size_t]),
             "orc_dispatch": (ctypes.c_int, [ctypes.c_int, P(d), vp]),
+            "orc_set_threads": (ctypes.c_int, [ctypes.c_int]),
             "orc_expect": (ctypes.c_int, [ctypes.c_int, P(d), ctypes.c_size_t, P(d), P(i32), P(i32), P(i32), P(d)]),
             "orc_probs": (ctypes.c_int, [ctypes.c_int, P(d), P(i32), ctypes.c_int, P(d)]),
             "orc_sample_from_probs": (u64, [P(d), ctypes.c_size_t, d]),
@@ -47,6 +48,9 @@ def lib():
             "orc_kraus": (ctypes.c_int, [vp, P(d), ctypes.c_int, P(ctypes.c_int), P(ctypes.c_int)]),
             "orc_superop": (ctypes.c_int, [vp, P(d), P(ctypes.c_int)]),
             "orc_dm_from_reg": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
+            "orc_deterministic_state": (ctypes.c_int, [ctypes.c_int, P(d)]),
+            "orc_rand_state": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
+            "orc_dm_mixed": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), P(d), P(d)]),
             "orc_dm_apply": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, P(d), vp, ctypes.c_size_t]),
             "orc_dm_trace": (ctypes.c_int, [ctypes.c_int, P(d), P(d)]),
             "orc_dm_purity": (d, [ctypes.c_int, P(d)]),
@@ -133,6 +137,12 @@ def instruct(fn, nbits, state, locs, gate=None, ctrl_locs=(), ctrl_bits=()):
     return _cx(s)
 
 
+def set_threads(k):
+    """Threads for the oracle's loops over independent amplitude groups (results do
+    not depend on it); returns the previous count. The default is 1."""
+    return lib().orc_set_threads(int(k))
+
+
 def apply(nbits, state, oplist, circuit_nbits=None):
     s = _il(state)
     _ck(lib().orc_apply(nbits, nbits if circuit_nbits is None else circuit_nbits, _d(s), oplist.arr, oplist.n))
@@ -212,6 +222,38 @@ def dm_from_reg(nbits, state):
     rho = np.zeros(2 << (2 * nbits))
     _ck(lib().orc_dm_from_reg(nbits, _d(_il(state)), _d(rho)))
     return _cx(rho)
+
+
+def deterministic_state(nbits):
+    """deterministic_state (register.rs:71-84)."""
+    out = np.zeros(2 << nbits)
+    _ck(lib().orc_deterministic_state(nbits, _d(out)))
+    return _cx(out)
+
+
+def rand_state(nbits, uniforms):
+    """rand_state (register.rs:35-45) from 2 * 2^nbits f64 uniforms (re, im per amplitude)."""
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    assert u.size == 2 << nbits
+    out = np.zeros(2 << nbits)
+    _ck(lib().orc_rand_state(nbits, _d(u), _d(out)))
+    return _cx(out)
+
+
+def dm_mixed(nbits, weights, states):
+    """DensityMatrix::mixed (density_matrix.rs:37-61)."""
+    w = np.ascontiguousarray(weights, dtype=np.float64)
+    st = np.concatenate([_il(s) for s in states])
+    rho = np.zeros(2 << (2 * nbits))
+    _ck(lib().orc_dm_mixed(nbits, len(states), _d(w), _d(st), _d(rho)))
+    return _cx(rho)
+
+
+def density_matrix_from_reg(nbits, state, locs):
+    """density_matrix_from_reg (density_matrix.rs:266-272): from_reg, then partial_tr
+    over the bit positions not in locs."""
+    traced = [l for l in range(nbits) if l not in locs]
+    return dm_partial_tr(nbits, dm_from_reg(nbits, state), traced)
 
 
 def dm_apply(nbits, rho, oplist):

@@ -1,10 +1,14 @@
 """Builders shared by the tests: circuits from the reference's JSON fixtures
-(mirroring tests/common/mod.rs:154-240 of yao-rs) and the benchmark workloads."""
+(mirroring tests/common/mod.rs:154-240 of yao-rs) and random circuits of every
+dispatch class. The benchmark workloads live in yaohip.workloads and are
+re-exported here."""
 import math
 
 import numpy as np
 
-from yaohip.circuit import (Gate, NoiseChannel, PositionedGate, channel, control, put, lower)
+from yaohip.circuit import Gate, PositionedGate, put
+from yaohip.workloads import (brickwork, brickwork_layers, noisy_brickwork, qft_no_swaps,  # noqa: F401
+                              qft_with_swaps)
 
 NAMED = {
     "X": lambda p: Gate.X, "Y": lambda p: Gate.Y, "Z": lambda p: Gate.Z, "H": lambda p: Gate.H,
@@ -43,57 +47,6 @@ def parse_state(pairs):
 
 def parse_matrix(rows):
     return np.array([[complex(a, b) for a, b in row] for row in rows], dtype=np.complex128)
-
-
-def qft_with_swaps(n):
-    """QFT with the final bit-reversal SWAPs (examples/qft.rs:30-54)."""
-    els = []
-    for i in range(n):
-        els.append(put([i], Gate.H))
-        for j in range(1, n - i):
-            els.append(control([i + j], [i], Gate.Phase(2.0 * math.pi / float(1 << (j + 1)))))
-    for i in range(n // 2):
-        els.append(PositionedGate(Gate.SWAP, [i, n - 1 - i]))
-    return els
-
-
-def qft_no_swaps(n):
-    """easybuild::qft_circuit (easybuild.rs:55-70)."""
-    els = []
-    for i in range(n):
-        els.append(put([i], Gate.H))
-        for j in range(1, n - i):
-            els.append(control([i + j], [i], Gate.Phase(2.0 * math.pi / float(1 << (j + 1)))))
-    return els
-
-
-def brickwork(n, depth, seed):
-    """Random brickwork: per layer Rx, Ry, Rz (angles U[0, 2pi)) on every qubit, then
-    CNOT(2i -> 2i+1) in even layers and CNOT(2i+1 -> 2i+2) in odd layers."""
-    rng = np.random.default_rng(seed)
-    els = []
-    for layer in range(depth):
-        for q in range(n):
-            a = rng.uniform(0.0, 2.0 * math.pi, size=3)
-            els.append(put([q], Gate.Rx(a[0])))
-            els.append(put([q], Gate.Ry(a[1])))
-            els.append(put([q], Gate.Rz(a[2])))
-        start = 0 if layer % 2 == 0 else 1
-        for c in range(start, n - 1, 2):
-            els.append(control([c], [c + 1], Gate.X))
-    return els
-
-
-def noisy_brickwork(n, depth, seed, p=1e-3, gamma=1e-3):
-    """Brickwork on a density matrix: after every gate, on each touched qubit, a
-    Depolarizing(p) then AmplitudeDamping(gamma) channel."""
-    out = []
-    for e in brickwork(n, depth, seed):
-        out.append(e)
-        for q in list(e.target_locs) + list(e.control_locs):
-            out.append(channel([q], NoiseChannel.Depolarizing(1, p)))
-            out.append(channel([q], NoiseChannel.AmplitudeDamping(gamma, 0.0)))
-    return out
 
 
 def random_circuit(n, ngates, seed, max_targets=3, max_ctrls=2, custom_targets=(1, 2, 3)):

@@ -254,3 +254,32 @@ def test_block_pass_stats_invariants():
         assert alone == plan[2]
         assert 0 < small < in_multi and din <= multi and dout <= multi
         assert in_multi <= 5 * multi
+
+
+def test_block_plan_cache_keys_on_structure():
+    """Plans are cached by the whole circuit structure: circuits with the same qubit
+    and element counts but different positions get their own plans, and a cached
+    plan is returned only for its own circuit (angles do not matter)."""
+    from fixture_circuits import brickwork
+    n = 16
+    a = brickwork(n, 3, 1)
+    b = [put([n - 1 - e.target_locs[0]], e.gate) if not e.control_locs else
+         control([n - 1 - e.control_locs[0]], [n - 1 - e.target_locs[0]], e.gate) for e in a]
+    a2 = brickwork(n, 3, 2)  # same structure, other angles
+
+    def plan(els):
+        ol = lower(els)
+        ns = ctypes.c_size_t()
+        A.check(A.lib().yh_block_plan(n, ol.arr, ol.n, None, 0, ctypes.byref(ns)))
+        bits = np.zeros(5 * ns.value, dtype=np.int32)
+        A.check(A.lib().yh_block_plan(n, ol.arr, ol.n, A.i32ptr(bits), ns.value, ctypes.byref(ns)))
+        return bits.reshape(-1, 5)
+
+    pa, pb = plan(a), plan(b)
+    assert not np.array_equal(pa, pb)
+    assert np.array_equal(plan(a), pa) and np.array_equal(plan(a2), pa) and np.array_equal(plan(b), pb)
+    # every gate of b lies inside a block of b's plan
+    blocks = [set(int(x) for x in r) for r in pb if (r >= 0).all()]
+    for e in b:
+        g = {n - 1 - l for l in list(e.target_locs) + list(e.control_locs)}
+        assert any(g <= s for s in blocks)

@@ -115,7 +115,7 @@ def test_dm_unitary_matches_pure(gpu):
     reg = gpu.ArrayReg.from_vec(n, psi)
     dm = gpu.DensityMatrix.from_reg(reg)
     dm.apply(Circuit.qubits(n, els))
-    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg, 0)
     out = reg.state_vec()
     assert rel_err(dm.state_data(), np.outer(out, out.conj()).reshape(-1)) < 1e-12
     assert abs(dm.purity() - 1) < 1e-12
@@ -207,5 +207,86 @@ def test_partial_tr_edge_cases(gpu):
         dm.partial_tr([-1])
     # entropy of the mixed 3-qubit matrix against numpy's eigvalsh
     ev = np.maximum(np.linalg.eigvalsh(rho), 0.0)
+    ev = ev[ev > 1e-15]
+    assert abs(dm.von_neumann_entropy() - float(-np.sum(ev * np.log(ev)))) < 1e-10
+
+
+def random_kraus(nq, nk, seed):
+    """nk Kraus operators on nq qubits with sum K^dagger K = I (a random isometry cut
+    into blocks)."""
+    rng = np.random.default_rng(seed)
+    d = 1 << nq
+    a = rng.standard_normal((nk * d, d)) + 1j * rng.standard_normal((nk * d, d))
+    q, _ = np.linalg.qr(a)
+    return [q[k * d:(k + 1) * d] for k in range(nk)]
+
+
+@pytest.mark.parametrize("fused", [False, True, "blocks"])
+def test_channels_on_three_or_more_qubits(gpu, fused):
+    """apply_channel (density_matrix.rs:181-224) for channels the 2-qubit kernels do not
+    cover: Depolarizing on 3 qubits (noise.rs:462-508, 64 Kraus operators), a
+    3-qubit Custom Kraus set, a 4-qubit Depolarizing channel (256 operators) and a
+    1-qubit Custom channel of 70 operators; against the oracle at 1e-12, trace kept."""
+    n = 6
+    psi = seeded_state(n, 71)
+    els = [put([q], Gate.H) for q in range(n)] + [control([0], [1], Gate.X), control([2], [4], Gate.X)]
+    els += [channel([1, 3, 5], NoiseChannel.Depolarizing(3, 0.2)), put([2], Gate.Ry(0.7))]
+    els += [channel([4, 0, 2], NoiseChannel.Custom(random_kraus(3, 5, 1))), control([3], [5], Gate.X)]
+    els += [channel([5, 1, 0, 3], NoiseChannel.Depolarizing(4, 0.1))]
+    els += [channel([2], NoiseChannel.Custom(random_kraus(1, 70, 2))), put([1], Gate.Rx(0.4))]
+    print(f"[case] dm channels >= 3 qubits fused={fused}", flush=True)
+    dm = gpu.DensityMatrix.from_reg(gpu.ArrayReg.from_vec(n, psi))
+    dm.apply(Circuit.qubits(n, els), fused=fused)
+    want = O.dm_apply(n, O.dm_from_reg(n, psi), lower(els))
+    assert rel_err(dm.state_data(), want) < 1e-12
+    assert abs(dm.trace() - 1) < 1e-12
+
+
+def test_channel_three_qubits_on_a_large_matrix(gpu):
+    """A 3-qubit depolarizing channel on a 12-qubit density matrix (2^24 entries): the
+    Kraus branch sum (noise.rs:462-508 weights 1 - p + p/64 on the identity and p/64
+    on the other Pauli strings) against the Pauli-twirl closed form
+    rho -> (1 - p) rho + p tr_S(rho) (x) I/8 on the channel's qubits S."""
+    n, p = 12, 0.3
+    els = [put([q], Gate.Ry(0.1 + 0.2 * q)) for q in range(n)] + [control([q], [q + 1], Gate.X) for q in range(n - 1)]
+    dm = gpu.DensityMatrix.zero_state(n)
+    dm.apply(Circuit.qubits(n, els))
+    rho = dm.state_data()
+    S = [2, 7, 9]
+    dm.apply(Circuit.qubits(n, [channel(S, NoiseChannel.Depolarizing(3, p))]))
+    got = dm.state_data()
+    # tensor axes: row of loc l is axis l, column of loc l is axis n + l
+    t = rho.reshape([2] * (2 * n))
+    lab = list(range(2 * n))
+    tr_in = [l if (l >= n and l - n in S) is False else l - n for l in lab]
+    out = [l for l in lab if l not in S and l - n not in S]
+    tr = np.einsum(t, tr_in, out)
+    eye = np.eye(8).reshape([2] * 6)
+    full = np.einsum(tr, out, eye, S + [n + l for l in S], lab) / 8
+    want = (1 - p) * rho + p * full.reshape(-1)
+    assert np.max(np.abs(got - want)) < 1e-12
+    assert abs(dm.trace() - 1) < 1e-12
+
+
+@pytest.mark.parametrize("n", [11, 12])
+def test_entropy_beyond_ten_qubits(gpu, n):
+    """von_neumann_entropy (density_matrix.rs:113-122) without a size cap: above 10
+    qubits the eigenvalues come from the device eigensolver; against numpy's eigvalsh
+    (which pins the oracle's eigensolver at small sizes) and the closed form of a
+    mixture of orthonormal states, -sum w ln w."""
+    rng = np.random.default_rng(n)
+    k = 6
+    w = rng.random(k)
+    w /= w.sum()
+    q, _ = np.linalg.qr(rng.standard_normal((1 << n, k)) + 1j * rng.standard_normal((1 << n, k)))
+    regs = [gpu.ArrayReg.from_vec(n, q[:, j]) for j in range(k)]
+    dm = gpu.DensityMatrix.mixed(w, regs)
+    s = dm.von_neumann_entropy()
+    want = float(-np.sum(w * np.log(w)))
+    print(f"[case] entropy n={n}: {s:.14f} closed form {want:.14f}", flush=True)
+    assert abs(s - want) < 1e-10
+    # the same matrix after noise, against numpy
+    dm.apply(Circuit.qubits(n, [channel([q], NoiseChannel.Depolarizing(1, 0.05)) for q in range(3)]))
+    ev = np.maximum(np.linalg.eigvalsh(dm.state_data().reshape(1 << n, 1 << n)), 0.0)
     ev = ev[ev > 1e-15]
     assert abs(dm.von_neumann_entropy() - float(-np.sum(ev * np.log(ev)))) < 1e-10

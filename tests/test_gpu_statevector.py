@@ -133,9 +133,9 @@ def test_brickwork_26q_round_trip_and_norm(gpu):
     els = brickwork(n, 2, 5)
     psi = seeded_state(n, 9)
     reg = gpu.ArrayReg.from_vec(n, psi)
-    gpu.apply_inplace(Circuit.qubits(n, els), reg)
+    gpu.apply_inplace(Circuit.qubits(n, els), reg, 0)
     assert abs(reg.norm() - 1.0) < 1e-12
-    gpu.apply_inplace(Circuit.qubits(n, dagger(els)), reg)
+    gpu.apply_inplace(Circuit.qubits(n, dagger(els)), reg, 0)
     assert rel_err(reg.state_vec(), psi) < 1e-12
 
 

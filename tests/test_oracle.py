@@ -389,3 +389,59 @@ def test_validation_errors():
     assert O.validate(2, lower([put([0], Gate.SWAP)])) == 5
     assert O.validate(2, lower([channel([0, 1], NoiseChannel.BitFlip(0.1))])) == 5
     assert O.validate(2, lower([channel([0, 0], NoiseChannel.Depolarizing(2, 0.1))])) == 4
+
+
+# ---------------- constructors (register.rs, density_matrix.rs) ----------------
+def test_deterministic_state_known_answers():
+    """register.rs:188-194: 4 qubits, 16 amplitudes, norm 1, amplitudes 0 and 1
+    differ; and the definition's closed forms: amplitude 0 is (1, 0) / norm, the
+    phase of amplitude k is atan2(sin 0.2k, cos 0.1k)."""
+    s = O.deterministic_state(4)
+    assert s.shape == (16,) and abs(np.linalg.norm(s) - 1) < 1e-12 and abs(s[0] - s[1]) > 1e-6
+    nrm = math.sqrt(sum(math.cos(0.1 * k) ** 2 + math.sin(0.2 * k) ** 2 for k in range(16)))
+    assert abs(s[0] - 1 / nrm) < 1e-15
+    assert abs(np.angle(s[7]) - math.atan2(math.sin(1.4), math.cos(0.7))) < 1e-14
+
+
+def test_rand_state_from_uniforms():
+    """rand_state (register.rs:35-45) with a fixed stream: constant uniforms 0.75 give
+    every amplitude (0.25 + 0.25i) before normalisation, i.e. the uniform state times
+    e^{i pi/4}; uniforms 0.5 in the imaginary slots give a real state."""
+    n = 5
+    s = O.rand_state(n, np.full(2 << n, 0.75))
+    np.testing.assert_allclose(s, np.exp(1j * math.pi / 4) / math.sqrt(1 << n), rtol=0, atol=1e-15)
+    u = np.random.default_rng(1).random(2 << n)
+    u[1::2] = 0.5
+    s = O.rand_state(n, u)
+    assert np.all(s.imag == 0) and abs(np.linalg.norm(s) - 1) < 1e-14
+    np.testing.assert_allclose(s.real * np.linalg.norm(u[0::2] - 0.5), u[0::2] - 0.5, rtol=0, atol=1e-15)
+
+
+def test_mixed_known_answers():
+    """density_matrix.rs:304-310 (purity 1/2 of an equal mixture of |0>, |1>),
+    :322-330 (entropy ln 2), tests/suites/measure.rs:610-619 (diagonal 0.25, 0.75)."""
+    e0, e1 = np.array([1, 0], dtype=complex), np.array([0, 1], dtype=complex)
+    r = O.dm_mixed(1, [0.5, 0.5], [e0, e1])
+    assert abs(O.dm_purity(1, r) - 0.5) < 1e-12
+    assert abs(O.dm_entropy(1, r.reshape(2, 2)) - math.log(2)) < 1e-8
+    np.testing.assert_allclose(O.dm_probs(1, O.dm_mixed(1, [0.25, 0.75], [e0, e1])), [0.25, 0.75], atol=1e-15)
+    # a one-register mixture with weight 1 is from_reg, bit for bit
+    psi = seeded_state(4, 3)
+    assert np.array_equal(O.dm_mixed(4, [1.0], [psi]), O.dm_from_reg(4, psi))
+
+
+def test_density_matrix_from_reg_known_answers():
+    """density_matrix_from_reg (density_matrix.rs:266-272) = partial_tr of from_reg:
+    keeping every bit is from_reg; one qubit of a Bell pair is maximally mixed
+    (:334-341 of the partial-trace test); for a product state the reduced matrix
+    of the kept bits is the product of their factors."""
+    psi = seeded_state(3, 5)
+    full = O.density_matrix_from_reg(3, psi, [0, 1, 2]).reshape(-1)
+    assert np.array_equal(full, O.dm_from_reg(3, psi))
+    bell = np.array([1, 0, 0, 1], dtype=complex) / math.sqrt(2)
+    red = O.density_matrix_from_reg(2, bell, [0])
+    np.testing.assert_allclose(red, np.eye(2) / 2, atol=1e-15)
+    a, b = seeded_state(1, 6), seeded_state(2, 7)
+    prod = np.kron(a, b)  # bits 2 (a) above bits 1, 0 (b)
+    red = O.density_matrix_from_reg(3, prod, [0, 1])
+    np.testing.assert_allclose(red, np.outer(b, b.conj()), atol=1e-15)

@@ -47,3 +47,15 @@ def test_io_errors(tmp_path):
         p = write(tmp_path, "h.bin", bad)
         with pytest.raises(A.YhError):
             yaohip.read_state_header(p)
+
+
+def test_deeply_nested_unknown_field(tmp_path):
+    """An unknown field nested far deeper than serde_json's recursion limit (128) is
+    rejected as a format error, not by overflowing the host stack."""
+    deep = b'{"format": "yao-state-v1", "note": ' + b"[" * 200000 + b"]" * 200000 + b"}\n"
+    p = write(tmp_path, "deep.bin", deep)
+    with pytest.raises(A.YhError) as e:
+        yaohip.read_state_header(p)
+    assert e.value.status == 16
+    ok = header_bytes(2, extra={"note": [[[[1]]]]}) + bytes(64)
+    assert yaohip.read_state_header(write(tmp_path, "ok.bin", ok)) == 2

@@ -3,9 +3,14 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
+
+#include <rocsolver/rocsolver.h>
+
+#include <type_traits>
 
 #include "host.h"
 #include "kernels.h"
@@ -305,7 +310,7 @@ template int apply_gates<float>(Workspace&, void*, int, const yh_op*, size_t);
 // Lower terms to device form. lsb_sites: sites are bit positions (expect_dm);
 // otherwise loc l is bit nbits-1-l. Sites beyond the register contribute the
 // operator's [0][0] entry (expect_dm reads bit 0 there) and are dropped.
-static int lower_terms(int nbits, const std::vector<Term>& terms, bool lsb_sites, std::vector<TermDev>& out,
+int lower_terms(int nbits, const std::vector<Term>& terms, bool lsb_sites, std::vector<TermDev>& out,
                        std::vector<cd>& coeffs) {
   out.clear();
   coeffs.clear();
@@ -400,7 +405,7 @@ int yh_copy_bandwidth(size_t bytes, int reps, double* gbps) {
 
 int yh_reg_create(int nbits, int dtype, yh_reg** out) {
   YH_GUARD(out, YH_ERR_INVALID_ARG, "null output pointer");
-  YH_GUARD(nbits >= 1 && nbits <= 36, YH_ERR_INVALID_ARG, "nbits must be in [1, 36]");
+  YH_GUARD(nbits >= 1 && nbits <= 35, YH_ERR_INVALID_ARG, "nbits must be in [1, 35]");
   YH_GUARD(dtype == YH_C128 || dtype == YH_C64, YH_ERR_DTYPE, "unknown dtype");
   YH_TRY(ensure_init());
   yh_reg* r = new yh_reg();
@@ -428,6 +433,92 @@ int yh_reg_create(int nbits, int dtype, yh_reg** out) {
   }
   *out = r;
   return YH_OK;
+}
+
+static int reg_norm2(yh_reg* reg, double* sumsq_dev);
+static int reg_scale(yh_reg* reg, const double* sumsq_dev, double thresh);
+
+// amplitude idx = (re, 0), written with one host-to-device copy on the register's stream
+static int reg_set_amp(yh_reg* r, uint64_t idx, double re) {
+  if (r->dtype == YH_C128) {
+    const double v[2] = {re, 0.0};
+    YH_HIP(hipMemcpyAsync(static_cast<double2*>(r->state.p) + idx, v, sizeof(v), hipMemcpyHostToDevice, r->w.stream));
+  } else {
+    const float v[2] = {float(re), 0.0f};
+    YH_HIP(hipMemcpyAsync(static_cast<float2*>(r->state.p) + idx, v, sizeof(v), hipMemcpyHostToDevice, r->w.stream));
+  }
+  return r->w.sync();
+}
+
+// a register of nbits qubits whose amplitudes `fill` writes on its stream
+static int reg_make(int nbits, int dtype, yh_reg** out, const std::function<int(yh_reg*)>& fill) {
+  YH_GUARD(out, YH_ERR_INVALID_ARG, "null output pointer");
+  YH_GUARD(nbits >= 1 && nbits <= 35, YH_ERR_INVALID_ARG, "nbits must be in [1, 35]");
+  YH_GUARD(dtype == YH_C128 || dtype == YH_C64, YH_ERR_DTYPE, "unknown dtype");
+  YH_TRY(ensure_init());
+  std::unique_ptr<yh_reg> r(new yh_reg());
+  r->nbits = nbits;
+  r->dtype = dtype;
+  YH_TRY(r->w.init());
+  YH_TRY(r->state.ensure(r->bytes()));
+  YH_TRY(fill(r.get()));
+  YH_TRY(r->w.sync());
+  *out = r.release();
+  return YH_OK;
+}
+
+// amplitude `idx` = (re, 0), the rest zero
+static int reg_set_onehot(yh_reg* r, uint64_t idx, double re) {
+  YH_HIP(hipMemsetAsync(r->state.p, 0, r->bytes(), r->w.stream));
+  return reg_set_amp(r, idx, re);
+}
+
+int yh_reg_product_state(int nbits, int dtype, uint64_t index, yh_reg** out) {
+  YH_GUARD(nbits >= 1 && nbits < 64 && index < (uint64_t(1) << nbits), YH_ERR_INVALID_ARG,
+           "product_state: the bit string does not fit in nbits");
+  return reg_make(nbits, dtype, out, [&](yh_reg* r) { return reg_set_onehot(r, index, 1.0); });
+}
+
+int yh_reg_uniform_state(int nbits, int dtype, yh_reg** out) {
+  return reg_make(nbits, dtype, out, [&](yh_reg* r) {
+    const double amp = 1.0 / std::sqrt(double(uint64_t(1) << nbits));
+    const uint64_t n = uint64_t(1) << nbits;
+    if (dtype == YH_C128) return launch_fill<double>(static_cast<double2*>(r->state.p), n, amp, r->w.stream);
+    return launch_fill<float>(static_cast<float2*>(r->state.p), n, amp, r->w.stream);
+  });
+}
+
+int yh_reg_ghz_state(int nbits, int dtype, yh_reg** out) {
+  return reg_make(nbits, dtype, out, [&](yh_reg* r) {
+    const double amp = 1.0 / std::sqrt(2.0);
+    YH_TRY(reg_set_onehot(r, 0, amp));
+    return reg_set_amp(r, (uint64_t(1) << nbits) - 1, amp);
+  });
+}
+
+int yh_reg_deterministic_state(int nbits, int dtype, yh_reg** out) {
+  return reg_make(nbits, dtype, out, [&](yh_reg* r) {
+    const uint64_t n = uint64_t(1) << nbits;
+    if (dtype == YH_C128) YH_TRY(launch_deterministic<double>(static_cast<double2*>(r->state.p), n, r->w.stream));
+    else YH_TRY(launch_deterministic<float>(static_cast<float2*>(r->state.p), n, r->w.stream));
+    YH_TRY(reg_norm2(r, r->w.result()));
+    return reg_scale(r, r->w.result(), 0.0);
+  });
+}
+
+int yh_reg_rand_state(int nbits, int dtype, const double* uniforms, size_t n_uniforms, yh_reg** out) {
+  YH_GUARD(uniforms, YH_ERR_INVALID_ARG, "null uniforms");
+  YH_GUARD(nbits >= 1 && nbits <= 35 && n_uniforms == (size_t(2) << nbits), YH_ERR_INVALID_ARG,
+           "rand_state: 2 * 2^nbits uniforms (re, im per amplitude)");
+  return reg_make(nbits, dtype, out, [&](yh_reg* r) {
+    const uint64_t n = uint64_t(1) << nbits;
+    YH_TRY(r->w.upload(uniforms, n_uniforms * sizeof(double), r->w.scratch));
+    const double* u = static_cast<const double*>(r->w.scratch.p);
+    if (dtype == YH_C128) YH_TRY(launch_from_uniforms<double>(static_cast<double2*>(r->state.p), u, n, r->w.stream));
+    else YH_TRY(launch_from_uniforms<float>(static_cast<float2*>(r->state.p), u, n, r->w.stream));
+    YH_TRY(reg_norm2(r, r->w.result()));
+    return reg_scale(r, r->w.result(), 0.0);
+  });
 }
 
 int yh_reg_destroy(yh_reg* reg) {
@@ -1113,6 +1204,56 @@ int yh_dm_from_reg(const yh_reg* reg, yh_dm** out) {
   return YH_OK;
 }
 
+int yh_dm_mixed(const double* weights, const yh_reg* const* regs, size_t nregs, yh_dm** out) {
+  YH_GUARD(out && weights && regs, YH_ERR_INVALID_ARG, "null argument");
+  YH_GUARD(nregs >= 1, YH_ERR_INVALID_ARG, "mixed: at least one register");
+  const int n = regs[0] ? regs[0]->nbits : 0;
+  std::vector<const double2*> ptrs(nregs);
+  for (size_t j = 0; j < nregs; ++j) {
+    YH_GUARD(regs[j], YH_ERR_INVALID_ARG, "null register");
+    YH_GUARD(regs[j]->nbits == n, YH_ERR_NBITS_MISMATCH, "mixed: registers of different sizes");
+    YH_GUARD(regs[j]->dtype == YH_C128, YH_ERR_DTYPE, "density matrices are complex128");
+    YH_HIP(hipStreamSynchronize(regs[j]->w.stream));
+    ptrs[j] = static_cast<const double2*>(regs[j]->state.p);
+  }
+  yh_dm* d = nullptr;
+  YH_TRY(yh_dm_create(n, &d));
+  std::unique_ptr<yh_dm> hold(d);
+  DevBuf dp, dw;
+  YH_TRY(d->w.upload(ptrs.data(), nregs * sizeof(void*), dp));
+  YH_TRY(d->w.upload(weights, nregs * sizeof(double), dw));
+  YH_TRY(launch_mixed(static_cast<double2*>(d->rho.p), d->dim(), static_cast<const double2* const*>(dp.p),
+                      static_cast<const double*>(dw.p), int(nregs), d->w.stream));
+  YH_TRY(d->w.sync());
+  *out = hold.release();
+  return YH_OK;
+}
+
+int yh_dm_from_reg_locs(const yh_reg* reg, const int32_t* locs, int nlocs, yh_dm** out) {
+  YH_GUARD(reg && out && nlocs >= 0 && (nlocs == 0 || locs), YH_ERR_INVALID_ARG, "null argument");
+  YH_GUARD(reg->dtype == YH_C128, YH_ERR_DTYPE, "density matrices are complex128");
+  // kept locs are bit positions, as partial_tr reads them (the traced ones are
+  // 0..nbits without locs, density_matrix.rs:266-272)
+  PtrArgs a{};
+  for (int l = 0; l < reg->nbits; ++l) {
+    bool keep = false;
+    for (int k = 0; k < nlocs; ++k) keep = keep || locs[k] == l;
+    if (keep) a.kept[a.nkept++] = l;
+    else a.tr[a.ntr++] = l;
+  }
+  YH_GUARD(a.nkept <= 17, YH_ERR_INVALID_ARG, "the reduced density matrix has more than 17 qubits");
+  std::unique_ptr<yh_dm> d(new yh_dm());
+  d->nbits = a.nkept;
+  YH_TRY(d->w.init());
+  YH_TRY(d->rho.ensure(d->bytes()));
+  YH_HIP(hipStreamSynchronize(reg->w.stream));
+  YH_TRY(launch_reduced<double>(static_cast<const double2*>(reg->state.p), a, static_cast<double2*>(d->rho.p),
+                                d->w.stream));
+  YH_TRY(d->w.sync());
+  *out = d.release();
+  return YH_OK;
+}
+
 int yh_dm_destroy(yh_dm* dm) {
   delete dm;
   return YH_OK;
@@ -1130,6 +1271,42 @@ int yh_dm_upload(yh_dm* dm, const double* rho, size_t n) {
   YH_HIP(hipMemcpy(dm->rho.p, rho, dm->bytes(), hipMemcpyHostToDevice));
   return YH_OK;
 }
+
+}  // extern "C"
+
+// Eigenvalues (ascending) of a density matrix on the device, for matrices too large
+// for the host path: rocSOLVER's divide-and-conquer Hermitian eigensolver (zheevd,
+// eigenvalues only) on a copy of rho, which it overwrites. The row-major rho read
+// as column-major is its transpose, conj(rho), which has the same eigenvalues.
+static int dm_eigenvalues_device(yh_dm* dm, std::vector<double>& ev) {
+  Workspace& w = dm->w;
+  const int d = int(dm->dim());
+  YH_TRY(w.acc.ensure(dm->bytes()));
+  DevBuf dv, ev_off, info;
+  YH_TRY(dv.ensure(sizeof(double) * size_t(d)));
+  YH_TRY(ev_off.ensure(sizeof(double) * size_t(d)));
+  YH_TRY(info.ensure(sizeof(rocblas_int)));
+  YH_HIP(hipMemcpyAsync(w.acc.p, dm->rho.p, dm->bytes(), hipMemcpyDeviceToDevice, w.stream));
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return fail(YH_ERR_HIP, "rocblas_create_handle failed");
+  std::unique_ptr<std::remove_pointer<rocblas_handle>::type, decltype(&rocblas_destroy_handle)> hold(
+      h, &rocblas_destroy_handle);
+  if (rocblas_set_stream(h, w.stream) != rocblas_status_success) return fail(YH_ERR_HIP, "rocblas_set_stream failed");
+  const rocblas_status st =
+      rocsolver_zheevd(h, rocblas_evect_none, rocblas_fill_lower, d, static_cast<rocblas_double_complex*>(w.acc.p), d,
+                       static_cast<double*>(dv.p), static_cast<double*>(ev_off.p), static_cast<rocblas_int*>(info.p));
+  if (st != rocblas_status_success)
+    return fail(st == rocblas_status_memory_error ? YH_ERR_OUT_OF_MEMORY : YH_ERR_HIP,
+                std::string("rocsolver_zheevd failed: ") + rocblas_status_to_string(st));
+  rocblas_int inf = 0;
+  ev.assign(size_t(d), 0.0);
+  YH_TRY(w.fetch(ev.data(), dv.p, sizeof(double) * size_t(d)));
+  YH_TRY(w.fetch(&inf, info.p, sizeof(inf)));
+  if (inf != 0) return fail(YH_ERR_HIP, "eigendecomposition failed (zheevd did not converge)");
+  return YH_OK;
+}
+
+extern "C" {
 
 int yh_dm_partial_tr(const yh_dm* cdm, const int32_t* traced, int ntraced, yh_dm** out) {
   YH_GUARD(cdm && out && (ntraced == 0 || traced) && ntraced >= 0, YH_ERR_INVALID_ARG, "null argument");
@@ -1162,15 +1339,20 @@ int yh_dm_partial_tr(const yh_dm* cdm, const int32_t* traced, int ntraced, yh_dm
 
 int yh_dm_entropy(const yh_dm* dm, double* out) {
   YH_GUARD(dm && out, YH_ERR_INVALID_ARG, "null argument");
-  YH_GUARD(dm->nbits <= 10, YH_ERR_UNSUPPORTED, "von Neumann entropy is computed for at most 10 qubits");
   const int d = int(dm->dim());
-  std::vector<double> buf(size_t(2) * d * d);
-  YH_TRY(yh_dm_download(dm, buf.data(), size_t(d) * d));
-  Mat m(size_t(d) * d);
-  for (size_t k = 0; k < m.size(); ++k) m[k] = cd{buf[2 * k], buf[2 * k + 1]};
-  // von_neumann_entropy (density_matrix.rs:113-122): eigenvalues clamped at 0,
-  // those above 1e-15 contribute -v ln v
-  std::vector<double> ev = hermitian_eigenvalues(m, d);
+  std::vector<double> ev;
+  if (dm->nbits <= 10) {
+    // small matrices: downloaded, eigenvalues on the host
+    std::vector<double> buf(size_t(2) * d * d);
+    YH_TRY(yh_dm_download(dm, buf.data(), size_t(d) * d));
+    Mat m(size_t(d) * d);
+    for (size_t k = 0; k < m.size(); ++k) m[k] = cd{buf[2 * k], buf[2 * k + 1]};
+    ev = hermitian_eigenvalues(m, d);
+  } else {
+    YH_TRY(dm_eigenvalues_device(const_cast<yh_dm*>(dm), ev));
+  }
+  // von_neumann_entropy (density_matrix.rs:113-122): eigenvalues clamped at 0
+  // (hermitian_eigenvalues :227-242), those above 1e-15 contribute -v ln v
   double s = 0.0;
   for (double v : ev) {
     v = std::max(v, 0.0);
@@ -1193,10 +1375,14 @@ int yh_dm_download(const yh_dm* dm, double* rho, size_t n) {
 namespace yh {
 
 struct DmLaunch {
-  bool block = true;  // false: two dense passes on the 2n-bit vector
+  // 0: one k_dm_block pass; 1: the passes in `sv` on the 2n-bit vector (a gate on
+  // more than 2 qubits: U on the row bits, conj(U) on the column bits); 2: a channel
+  // as a sum of Kraus branches, branch k being the passes in branches[k]
+  int mode = 0;
   int K = 1;
   DmArgs a{};
-  std::vector<Launch<double>> sv;  // fallback passes
+  std::vector<Launch<double>> sv;
+  std::vector<std::vector<Launch<double>>> branches;
 };
 
 static void dm_block_geometry(int n, const std::vector<int>& qubits, DmArgs& a) {
@@ -1237,13 +1423,50 @@ static Mat controlled_block_matrix(const Mat& u, int nt, const yh_op& op) {
   return m;
 }
 
+// A channel on more than 2 qubits, or with more Kraus operators than the block
+// kernel folds in registers, as apply_channel (density_matrix.rs:181-224) does it:
+// per Kraus operator K (in order) a branch copy of rho, K on the row bits and
+// conj(K) on the column bits (dispatch_arrayreg_gate of a Custom gate, diagonal when
+// K is), then the branch added into the accumulated sum.
+static int dm_plan_kraus_branches(int n, const yh_op& op, const std::vector<Mat>& kr, int d, std::vector<DmLaunch>& L,
+                                  Arena<double>& arena) {
+  DmLaunch l;
+  l.mode = 2;
+  for (const Mat& K : kr) {
+    const int dg = is_diagonal_matrix(K, d) ? 1 : 0;
+    std::vector<double> mk(2 * K.size()), mc(2 * K.size());
+    for (size_t k = 0; k < K.size(); ++k) {
+      mk[2 * k] = mc[2 * k] = K[k].re;
+      mk[2 * k + 1] = K[k].im;
+      mc[2 * k + 1] = -K[k].im;
+    }
+    yh_op row{}, col{};
+    row.elem = col.elem = YH_ELEM_GATE;
+    row.kind = col.kind = YH_GATE_CUSTOM;
+    row.nt = col.nt = op.nt;
+    for (int k = 0; k < op.nt; ++k) {
+      row.targets[k] = op.targets[k];
+      col.targets[k] = op.targets[k] + n;
+    }
+    row.dim = col.dim = d;
+    row.is_diag = col.is_diag = dg;
+    row.matrix = mk.data();
+    col.matrix = mc.data();
+    std::vector<Launch<double>> br;
+    YH_TRY(plan_gate<double>(2 * n, row, br, arena));
+    YH_TRY(plan_gate<double>(2 * n, col, br, arena));
+    l.branches.push_back(std::move(br));
+  }
+  L.push_back(std::move(l));
+  return YH_OK;
+}
+
 static int dm_plan_channel(int n, const yh_op& op, std::vector<DmLaunch>& L, Arena<double>& arena) {
   std::vector<Mat> kr;
   int d = 0;
   int st = kraus_operators(op, kr, d);
   if (st != YH_OK) return fail(st, "invalid channel parameters");
-  if (op.nt > 2) return fail(YH_ERR_UNSUPPORTED, "density-matrix channels act on at most 2 qubits here");
-  if (kr.size() > 64) return fail(YH_ERR_UNSUPPORTED, "too many Kraus operators");
+  if (op.nt > 2 || kr.size() > 64) return dm_plan_kraus_branches(n, op, kr, d, L, arena);
   DmLaunch l;
   l.K = op.nt;
   std::vector<int> q(op.targets, op.targets + op.nt);
@@ -1280,7 +1503,7 @@ static int dm_plan_gate(int n, const yh_op& op, std::vector<DmLaunch>& L, Arena<
   }
   // nt > 2: U on the row bits, conj(U) on the column bits, as two dense passes
   DmLaunch l;
-  l.block = false;
+  l.mode = 1;
   yh_op row = op, col = op;
   row.kind = col.kind = YH_GATE_CUSTOM;
   std::vector<double> mu(2 * u.size()), mc(2 * u.size());
@@ -1360,36 +1583,60 @@ static int dm_plan_fused(int n, const yh_op* ops, size_t nops, std::vector<DmLau
   return YH_OK;
 }
 
+// per-gate passes on a 2n-bit vector `buf` (a copy for an out-of-place pass goes to w.scratch)
+static int dm_run_sv(Workspace& w, double2* buf, int nbits2, size_t bytes, const std::vector<Launch<double>>& sv,
+                     const unsigned char* base) {
+  for (auto& s : sv) {
+    switch (s.type) {
+      case L_1Q: YH_TRY(launch_1q<double>(buf, nbits2, s.bit, s.m2, w.stream)); break;
+      case L_DENSE:
+        YH_TRY(launch_dense<double>(buf, s.ga, s.nt, reinterpret_cast<const double2*>(base + s.moff), w.stream));
+        break;
+      case L_DIAG: YH_TRY(launch_diag<double>(buf, s.ga, s.nt, s.dg, w.stream)); break;
+      case L_X: YH_TRY(launch_xflip<double>(buf, s.ga, s.m0, w.stream)); break;
+      case L_SWAP: YH_TRY(launch_swap<double>(buf, s.ga, s.m0, s.m1, w.stream)); break;
+      case L_OOP: {
+        YH_TRY(w.scratch.ensure(bytes));
+        YH_HIP(hipMemcpyAsync(w.scratch.p, buf, bytes, hipMemcpyDeviceToDevice, w.stream));
+        YH_TRY(launch_dense_oop<double>(buf, static_cast<const double2*>(w.scratch.p), s.ga, s.nt,
+                                        reinterpret_cast<const double2*>(base + s.moff),
+                                        reinterpret_cast<const uint64_t*>(base + s.ooff), w.stream));
+        break;
+      }
+    }
+  }
+  return YH_OK;
+}
+
 static int dm_run(yh_dm* dm, std::vector<DmLaunch>& L, Arena<double>& arena) {
   Workspace& w = dm->w;
   YH_TRY(w.upload(arena.bytes.data(), arena.bytes.size(), w.mats));
   const unsigned char* base = static_cast<const unsigned char*>(w.mats.p);
-  double2* rho = static_cast<double2*>(dm->rho.p);
+  const size_t bytes = dm->bytes();
   for (auto& l : L) {
-    if (l.block) {
+    double2* rho = static_cast<double2*>(dm->rho.p);
+    if (l.mode == 0) {
       YH_TRY(w.time_begin(YH_K_DM));
       YH_TRY(launch_dm_block(rho, l.a, l.K, reinterpret_cast<const double2*>(base), w.stream));
       YH_TRY(w.time_end());
+    } else if (l.mode == 1) {
+      YH_TRY(dm_run_sv(w, rho, 2 * dm->nbits, bytes, l.sv, base));
     } else {
-      for (auto& s : l.sv) {
-        switch (s.type) {
-          case L_1Q: YH_TRY(launch_1q<double>(rho, 2 * dm->nbits, s.bit, s.m2, w.stream)); break;
-          case L_DENSE:
-            YH_TRY(launch_dense<double>(rho, s.ga, s.nt, reinterpret_cast<const double2*>(base + s.moff), w.stream));
-            break;
-          case L_DIAG: YH_TRY(launch_diag<double>(rho, s.ga, s.nt, s.dg, w.stream)); break;
-          case L_X: YH_TRY(launch_xflip<double>(rho, s.ga, s.m0, w.stream)); break;
-          case L_SWAP: YH_TRY(launch_swap<double>(rho, s.ga, s.m0, s.m1, w.stream)); break;
-          case L_OOP: {
-            YH_TRY(w.scratch.ensure(dm->bytes()));
-            YH_HIP(hipMemcpyAsync(w.scratch.p, rho, dm->bytes(), hipMemcpyDeviceToDevice, w.stream));
-            YH_TRY(launch_dense_oop<double>(rho, static_cast<const double2*>(w.scratch.p), s.ga, s.nt,
-                                            reinterpret_cast<const double2*>(base + s.moff),
-                                            reinterpret_cast<const uint64_t*>(base + s.ooff), w.stream));
-            break;
-          }
-        }
+      // the sum of the branches builds up in w.acc (the first branch is computed
+      // there directly), each later branch in w.branch; rho stays the original
+      // until the sum is copied over it (callers such as the block path hold the
+      // rho pointer across elements, so it never changes)
+      YH_TRY(w.acc.ensure(bytes));
+      for (size_t k = 0; k < l.branches.size(); ++k) {
+        DevBuf& dst = k == 0 ? w.acc : w.branch;
+        YH_TRY(dst.ensure(bytes));
+        YH_HIP(hipMemcpyAsync(dst.p, rho, bytes, hipMemcpyDeviceToDevice, w.stream));
+        YH_TRY(dm_run_sv(w, static_cast<double2*>(dst.p), 2 * dm->nbits, bytes, l.branches[k], base));
+        if (k > 0)
+          YH_TRY(launch_add_inplace(static_cast<double2*>(w.acc.p), static_cast<const double2*>(w.branch.p),
+                                    bytes / sizeof(double2), w.stream));
       }
+      YH_HIP(hipMemcpyAsync(rho, w.acc.p, bytes, hipMemcpyDeviceToDevice, w.stream));
     }
   }
   return w.sync();
@@ -1468,7 +1715,6 @@ int yh_dm_apply_blocks(yh_dm* dm, int circuit_nbits, const yh_op* ops, size_t no
       std::vector<Mat> kr;
       int d = 0;
       if (kraus_operators(ops[e], kr, d) != YH_OK) return fail(YH_ERR_CHANNEL_PARAM, "invalid channel parameters");
-      if (ops[e].nt > 2) return fail(YH_ERR_UNSUPPORTED, "density-matrix channels act on at most 2 qubits here");
     }
   return apply_dm_blocks(dm->w, dm->rho.p, dm->nbits, ops, nops, [&](size_t e) {
     std::vector<DmLaunch> L;

@@ -743,28 +743,41 @@ struct Planner {
   }
 };
 
-// structural key of a circuit: everything but gate parameters and matrix values
-uint64_t structure_key(int nbits, const yh_op* ops, size_t nops) {
-  uint64_t h = 1469598103934665603ull ^ uint64_t(nbits);
-  auto mix = [&](uint64_t v) {
-    h ^= v;
-    h *= 1099511628211ull;
-  };
+// structural signature of a circuit: everything but gate parameters and matrix
+// values (element kind, gate or channel kind, positions, control values), which is
+// all the planner reads; `salt` separates plans of different planners
+std::vector<int32_t> structure_sig(int nbits, const yh_op* ops, size_t nops, int32_t salt) {
+  std::vector<int32_t> sig{salt, int32_t(nbits), int32_t(nops)};
   for (size_t e = 0; e < nops; ++e) {
     const yh_op& o = ops[e];
-    mix(uint64_t(o.elem) << 40 | uint64_t(uint32_t(o.kind)) << 8 | uint64_t(o.nt) << 4 | uint64_t(o.nc));
-    for (int k = 0; k < o.nt && k < YH_MAX_TARGETS; ++k) mix(uint64_t(o.targets[k]));
-    for (int k = 0; k < o.nc && k < YH_MAX_CTRLS; ++k) mix(uint64_t(o.ctrls[k]) << 1 | uint64_t(o.ctrl_vals[k] & 1));
+    sig.push_back(o.elem);
+    sig.push_back(o.kind);
+    sig.push_back(o.nt);
+    sig.push_back(o.nc);
+    sig.push_back(o.elem == YH_ELEM_CHANNEL ? o.nmat : 0);
+    sig.push_back(o.elem == YH_ELEM_CHANNEL ? int32_t(o.params[1]) : 0);  // depolarizing width
+    for (int k = 0; k < o.nt && k < YH_MAX_TARGETS; ++k) sig.push_back(o.targets[k]);
+    for (int k = 0; k < o.nc && k < YH_MAX_CTRLS; ++k) sig.push_back(o.ctrls[k] * 2 + (o.ctrl_vals[k] & 1));
   }
-  return h ^ nops;
+  return sig;
+}
+
+// 64-bit FNV-1a of a signature: a fast pre-filter for cache lookups; a hit also
+// compares the whole signature
+uint64_t sig_hash(const std::vector<int32_t>& sig) {
+  uint64_t h = 1469598103934665603ull;
+  for (int32_t v : sig) {
+    h ^= uint64_t(uint32_t(v));
+    h *= 1099511628211ull;
+  }
+  return h;
 }
 
 struct PlanCache {
   std::mutex mu;
   struct Entry {
     uint64_t key;
-    int nbits;
-    size_t nops;
+    std::vector<int32_t> sig;
     std::vector<BStep> steps;
   };
   std::vector<Entry> entries;  // most recent last, at most 8
@@ -774,16 +787,17 @@ PlanCache& plan_cache() {
   return c;
 }
 
-// fewest-blocks plan of a gate list, cached by `key` (the structure of the ops the
+// lowest-cost plan of a gate list, cached by `sig` (the structure of the ops the
 // gate list was built from)
 double plan_cost(const std::vector<BStep>& steps, int nbits);
 
-int plan_gates(const std::vector<BGate>& bg, int nbits, uint64_t key, size_t nops, std::vector<BStep>& steps) {
+int plan_gates(const std::vector<BGate>& bg, int nbits, const std::vector<int32_t>& sig, std::vector<BStep>& steps) {
+  const uint64_t key = sig_hash(sig);
   {
     PlanCache& c = plan_cache();
     std::lock_guard<std::mutex> lk(c.mu);
     for (auto& e : c.entries)
-      if (e.key == key && e.nbits == nbits && e.nops == nops) {
+      if (e.key == key && e.sig == sig) {
         steps = e.steps;
         return YH_OK;
       }
@@ -810,14 +824,14 @@ int plan_gates(const std::vector<BGate>& bg, int nbits, uint64_t key, size_t nop
   steps = bestp;
   PlanCache& c = plan_cache();
   std::lock_guard<std::mutex> lk(c.mu);
-  c.entries.push_back({key, nbits, nops, std::move(bestp)});
+  c.entries.push_back({key, sig, std::move(bestp)});
   if (c.entries.size() > 8) c.entries.erase(c.entries.begin());
   return YH_OK;
 }
 
 int plan_blocks(int nbits, const yh_op* ops, size_t nops, std::vector<BStep>& steps, std::vector<BGate>& bg) {
   YH_TRY(to_bgates(nbits, ops, nops, bg));
-  return plan_gates(bg, nbits, structure_key(nbits, ops, nops), nops, steps);
+  return plan_gates(bg, nbits, structure_sig(nbits, ops, nops, 0), steps);
 }
 
 // ---- density matrices: rho as a vector of 2n bits (row bits n..2n-1 above the
@@ -1290,7 +1304,7 @@ int apply_dm_blocks(Workspace& w, void* rho, int n, const yh_op* ops, size_t nop
     return YH_OK;
   }
   std::vector<BStep> steps;
-  YH_TRY(plan_gates(bg, 2 * n, structure_key(2 * n, ops, nops) ^ 0x9e3779b97f4a7c15ull, nops, steps));
+  YH_TRY(plan_gates(bg, 2 * n, structure_sig(2 * n, ops, nops, 1), steps));
   return run_block_plan<double>(w, rho, 2 * n, steps, bg, [&](const BGate& g) { return single_op(g.op_index); });
 }
 
@@ -1346,7 +1360,7 @@ int block_pass_stats(int nbits, const yh_op* ops, size_t nops, int dm, int64_t o
       out[6] = int64_t(bg.size());
       return YH_OK;
     }
-    YH_TRY(plan_gates(bg, nb, structure_key(nb, ops, nops) ^ 0x9e3779b97f4a7c15ull, nops, steps));
+    YH_TRY(plan_gates(bg, nb, structure_sig(nb, ops, nops, 1), steps));
   } else {
     if (nbits < kBQ) return fail(YH_ERR_UNSUPPORTED, "fewer qubits than a block");
     YH_TRY(plan_blocks(nbits, ops, nops, steps, bg));

@@ -88,6 +88,17 @@ __device__ __forceinline__ void lds_barrier() {
 constexpr int kMaxLocked = 32;
 constexpr int kBlock = 256;
 
+// Kernels that give each thread one item (or a fixed number of items) walk their
+// items with a grid stride, over at most kMaxGridBlocks workgroups: HIP refuses a
+// launch of 2^32 or more work-items in one dimension, which a 33-qubit state with one
+// thread per amplitude would need. Grids of states up to 30 qubits are unchanged.
+constexpr uint64_t kMaxGridBlocks = uint64_t(1) << 20;
+inline unsigned capped_grid(uint64_t work, uint64_t per_block) {
+  uint64_t b = (work + per_block - 1) / per_block;
+  if (b < 1) b = 1;
+  return unsigned(b > kMaxGridBlocks ? kMaxGridBlocks : b);
+}
+
 struct LockedBits {
   uint8_t pos[kMaxLocked];  // ascending bit positions locked to zero
   int32_t npos;

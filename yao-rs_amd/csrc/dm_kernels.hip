@@ -143,25 +143,25 @@ template <int K>
 __global__ __launch_bounds__(kBlock) void k_dm_block(double2* __restrict__ rho, DmArgs a,
                                                      const double2* __restrict__ mats) {
   constexpr int B = 1 << K;
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= a.ngroups) return;
-  const uint64_t base = insert_zeros(g, a.lk.pos, a.lk.npos);
-  Blk<B> x;
-#pragma unroll
-  for (int r = 0; r < B; ++r)
-#pragma unroll
-    for (int c = 0; c < B; ++c) x.a[r][c] = rho[base + a.roff[r] + a.coff[c]];
-  const bool row_ok = (base & a.rc_mask) == a.rc_val;
-  const bool col_ok = (base & a.cc_mask) == a.cc_val;
-  for (int k = 0; k < a.nmicro; ++k) {
-    // control predicates bind the first micro-op only (the gate)
-    run_micro<B>(x, a.micro[k], mats, k == 0 ? row_ok : true, k == 0 ? col_ok : true);
-  }
-  if (a.nmicro > 1 || row_ok || col_ok) {
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < a.ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t base = insert_zeros(g, a.lk.pos, a.lk.npos);
+    Blk<B> x;
 #pragma unroll
     for (int r = 0; r < B; ++r)
 #pragma unroll
-      for (int c = 0; c < B; ++c) rho[base + a.roff[r] + a.coff[c]] = x.a[r][c];
+      for (int c = 0; c < B; ++c) x.a[r][c] = rho[base + a.roff[r] + a.coff[c]];
+    const bool row_ok = (base & a.rc_mask) == a.rc_val;
+    const bool col_ok = (base & a.cc_mask) == a.cc_val;
+    for (int k = 0; k < a.nmicro; ++k) {
+      // control predicates bind the first micro-op only (the gate)
+      run_micro<B>(x, a.micro[k], mats, k == 0 ? row_ok : true, k == 0 ? col_ok : true);
+    }
+    if (a.nmicro > 1 || row_ok || col_ok) {
+#pragma unroll
+      for (int r = 0; r < B; ++r)
+#pragma unroll
+        for (int c = 0; c < B; ++c) rho[base + a.roff[r] + a.coff[c]] = x.a[r][c];
+    }
   }
 }
 
@@ -253,16 +253,17 @@ __global__ __launch_bounds__(kBlock) void k_fold2(const double* __restrict__ par
   }
 }
 
-// rho[r][c] = psi[r] * conj(psi[c]) (DensityMatrix::from_reg, density_matrix.rs:23-35)
+// rho[r][c] = psi[r] * conj(psi[c]) (DensityMatrix::from_reg, density_matrix.rs:23-35),
+// unfused like the reference's products
 __global__ __launch_bounds__(kBlock) void k_dm_outer(const double2* __restrict__ psi, double2* __restrict__ rho,
                                                      uint64_t dim) {
-  uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (e >= dim * dim) return;
-  const uint64_t r = e / dim, c = e % dim;
-  const double2 a = psi[r];
-  double2 b = psi[c];
-  b.y = -b.y;
-  rho[e] = cmul(a, b);
+#pragma clang fp contract(off)
+  for (uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x; e < dim * dim; e += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t r = e / dim, c = e % dim;
+    const double2 a = psi[r], b = psi[c];
+    // a * conj(b), each product rounded as num-complex computes it
+    rho[e] = double2{a.x * b.x - a.y * (-b.y), a.x * (-b.y) + a.y * b.x};
+  }
 }
 
 // partial_tr (density_matrix.rs:80-111): out[rr][rc] = sum over the traced bits t
@@ -273,37 +274,50 @@ __global__ __launch_bounds__(kBlock) void k_dm_outer(const double2* __restrict__
 __global__ __launch_bounds__(kBlock) void k_partial_tr(const double2* __restrict__ rho, uint64_t dim,
                                                        PtrArgs a, double2* __restrict__ out) {
   const uint64_t dr = uint64_t(1) << a.nkept;
-  const uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (e >= dr * dr) return;
-  const uint64_t rr = e / dr, rc = e % dr;
-  uint64_t rb = 0, cb = 0;
-  for (int k = 0; k < a.nkept; ++k) {
-    rb |= ((rr >> k) & 1) << a.kept[k];
-    cb |= ((rc >> k) & 1) << a.kept[k];
+  for (uint64_t e = uint64_t(blockIdx.x) * kBlock + threadIdx.x; e < dr * dr; e += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t rr = e / dr, rc = e % dr;
+    uint64_t rb = 0, cb = 0;
+    for (int k = 0; k < a.nkept; ++k) {
+      rb |= ((rr >> k) & 1) << a.kept[k];
+      cb |= ((rc >> k) & 1) << a.kept[k];
+    }
+    const uint64_t nt = uint64_t(1) << a.ntr;
+    double sr = 0.0, si = 0.0;
+    for (uint64_t t = 0; t < nt; ++t) {
+      uint64_t tb = 0;
+      for (int k = 0; k < a.ntr; ++k) tb |= ((t >> k) & 1) << a.tr[k];
+      const double2 v = rho[(rb | tb) * dim + (cb | tb)];
+      sr += v.x;
+      si += v.y;
+    }
+    out[e] = double2{sr, si};
   }
-  const uint64_t nt = uint64_t(1) << a.ntr;
-  double sr = 0.0, si = 0.0;
-  for (uint64_t t = 0; t < nt; ++t) {
-    uint64_t tb = 0;
-    for (int k = 0; k < a.ntr; ++k) tb |= ((t >> k) & 1) << a.tr[k];
-    const double2 v = rho[(rb | tb) * dim + (cb | tb)];
-    sr += v.x;
-    si += v.y;
+}
+
+__global__ __launch_bounds__(kBlock) void k_add_inplace(double2* __restrict__ acc, const double2* __restrict__ b,
+                                                        uint64_t n) {
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+    const double2 x = acc[i], y = b[i];
+    acc[i] = double2{x.x + y.x, x.y + y.y};
   }
-  out[e] = double2{sr, si};
 }
 
 }  // namespace
 
+int launch_add_inplace(double2* acc, const double2* b, uint64_t n, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_inplace, dim3(capped_grid(n, kBlock)), dim3(kBlock), 0, st, acc, b, n);
+  return YH_OK;
+}
+
 int launch_partial_tr(const double2* rho, uint64_t dim, const PtrArgs& a, double2* out, hipStream_t st) {
   const uint64_t n = (uint64_t(1) << a.nkept) << a.nkept;
-  hipLaunchKernelGGL(k_partial_tr, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, rho, dim, a, out);
+  hipLaunchKernelGGL(k_partial_tr, dim3(capped_grid(n, kBlock)), dim3(kBlock), 0, st, rho, dim, a, out);
   YH_HIP(hipGetLastError());
   return YH_OK;
 }
 
 int launch_dm_block(double2* rho, const DmArgs& a, int K, const double2* mats, hipStream_t st) {
-  dim3 grid(unsigned((a.ngroups + kBlock - 1) / kBlock)), block(kBlock);
+  dim3 grid(capped_grid(a.ngroups, kBlock)), block(kBlock);
   if (K == 1) hipLaunchKernelGGL(k_dm_block<1>, grid, block, 0, st, rho, a, mats);
   else if (K == 2) hipLaunchKernelGGL(k_dm_block<2>, grid, block, 0, st, rho, a, mats);
   else return fail(YH_ERR_UNSUPPORTED, "density-matrix block kernel supports 1 or 2 qubits");
@@ -328,7 +342,7 @@ int launch_dm_purity(const double2* rho, uint64_t dim, double* partial, double* 
 
 int launch_dm_outer(const double2* psi, double2* rho, uint64_t dim, hipStream_t st) {
   uint64_t n = dim * dim;
-  hipLaunchKernelGGL(k_dm_outer, dim3(unsigned((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, psi, rho, dim);
+  hipLaunchKernelGGL(k_dm_outer, dim3(capped_grid(n, kBlock)), dim3(kBlock), 0, st, psi, rho, dim);
   return YH_OK;
 }
 

@@ -139,5 +139,17 @@ struct PtrArgs {
 };
 int launch_partial_tr(const double2* rho, uint64_t dim, const PtrArgs& a, double2* out, hipStream_t st);
 int launch_dm_outer(const double2* psi, double2* rho, uint64_t dim, hipStream_t st);
+// ---- init_kernels.hip ----
+template <class R>
+int launch_deterministic(typename Cplx<R>::V* s, uint64_t n, hipStream_t st);
+template <class R>
+int launch_from_uniforms(typename Cplx<R>::V* s, const double* u, uint64_t n, hipStream_t st);
+template <class R>
+int launch_fill(typename Cplx<R>::V* s, uint64_t n, double re, hipStream_t st);
+int launch_mixed(double2* rho, uint64_t dim, const double2* const* psi, const double* w, int nreg, hipStream_t st);
+template <class R>
+int launch_reduced(const typename Cplx<R>::V* psi, const PtrArgs& a, double2* out, hipStream_t st);
+// acc[i] += b[i] for n complex128 entries (the branch accumulation of apply_channel)
+int launch_add_inplace(double2* acc, const double2* b, uint64_t n, hipStream_t st);
 
 }  // namespace yh

@@ -182,10 +182,10 @@ __global__ __launch_bounds__(kBlock) void k_dm_expect(const double2* __restrict_
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_probs_full(const typename Cplx<R>::V* __restrict__ s, uint64_t n,
                                                        double* __restrict__ out) {
-  uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double re = s[i].x, im = s[i].y;
-  out[i] = norm_sqr_exact(re, im);
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) {
+    const double re = s[i].x, im = s[i].y;
+    out[i] = norm_sqr_exact(re, im);
+  }
 }
 
 __device__ __forceinline__ uint64_t bucket_base(uint64_t m, const int32_t* locbits, int nlocs) {
@@ -201,40 +201,40 @@ __global__ __launch_bounds__(kBlock) void k_probs_marginal(const typename Cplx<R
                                                            uint64_t dim, LockedBits lk, const int32_t* locbits,
                                                            int nlocs, uint64_t nbuckets, uint64_t nchunks,
                                                            uint64_t chunk_len, double* __restrict__ ws) {
-  uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (t >= nbuckets * nchunks) return;
-  const uint64_t m = t / nchunks, c = t % nchunks;
-  const uint64_t base = bucket_base(m, locbits, nlocs);
-  double acc = 0.0;
-  const uint64_t k0 = c * chunk_len, k1 = k0 + chunk_len;
-  for (uint64_t k = k0; k < k1; ++k) {
-    const uint64_t b = insert_zeros(k, lk.pos, lk.npos) | base;
-    double p;
-    if (DM) {
-      p = fmax(double(s[b * dim + b].x), 0.0);
-    } else {
-      const double re = s[b].x, im = s[b].y;
-      p = norm_sqr_exact(re, im);
+  for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < nbuckets * nchunks;
+       t += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t m = t / nchunks, c = t % nchunks;
+    const uint64_t base = bucket_base(m, locbits, nlocs);
+    double acc = 0.0;
+    const uint64_t k0 = c * chunk_len, k1 = k0 + chunk_len;
+    for (uint64_t k = k0; k < k1; ++k) {
+      const uint64_t b = insert_zeros(k, lk.pos, lk.npos) | base;
+      double p;
+      if (DM) {
+        p = fmax(double(s[b * dim + b].x), 0.0);
+      } else {
+        const double re = s[b].x, im = s[b].y;
+        p = norm_sqr_exact(re, im);
+      }
+      acc = __dadd_rn(acc, p);
     }
-    acc = __dadd_rn(acc, p);
+    ws[t] = acc;
   }
-  ws[t] = acc;
 }
 
 __global__ __launch_bounds__(kBlock) void k_chunk_fold(const double* __restrict__ ws, uint64_t nbuckets,
                                                        uint64_t nchunks, double* __restrict__ out) {
-  uint64_t m = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (m >= nbuckets) return;
-  double acc = 0.0;
-  for (uint64_t c = 0; c < nchunks; ++c) acc = __dadd_rn(acc, ws[m * nchunks + c]);
-  out[m] = acc;
+  for (uint64_t m = uint64_t(blockIdx.x) * kBlock + threadIdx.x; m < nbuckets; m += uint64_t(gridDim.x) * kBlock) {
+    double acc = 0.0;
+    for (uint64_t c = 0; c < nchunks; ++c) acc = __dadd_rn(acc, ws[m * nchunks + c]);
+    out[m] = acc;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_dm_diag_probs(const double2* __restrict__ rho, uint64_t dim,
                                                           double* __restrict__ out) {
-  uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= dim) return;
-  out[i] = fmax(rho[i * dim + i].x, 0.0);
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < dim; i += uint64_t(gridDim.x) * kBlock)
+    out[i] = fmax(rho[i * dim + i].x, 0.0);
 }
 
 // Sequential cumulative sum in the reference's order (measure.rs:140-150).
@@ -262,49 +262,48 @@ __global__ void k_cdf(const double* __restrict__ p, uint64_t n, double* __restri
 __global__ __launch_bounds__(kBlock) void k_search(const double* __restrict__ cdf, uint64_t n,
                                                    const double* __restrict__ r, uint64_t nshots,
                                                    uint64_t* __restrict__ out) {
-  uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (k >= nshots) return;
-  const double x = r[k];
-  uint64_t lo = 0, hi = n;  // invariant: answer in [lo, hi]; hi == n means "none"
-  while (lo < hi) {
-    uint64_t mid = lo + (hi - lo) / 2;
-    if (x < cdf[mid]) hi = mid;
-    else lo = mid + 1;
+  for (uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x; k < nshots; k += uint64_t(gridDim.x) * kBlock) {
+    const double x = r[k];
+    uint64_t lo = 0, hi = n;  // invariant: answer in [lo, hi]; hi == n means "none"
+    while (lo < hi) {
+      uint64_t mid = lo + (hi - lo) / 2;
+      if (x < cdf[mid]) hi = mid;
+      else lo = mid + 1;
+    }
+    out[k] = lo < n ? lo : n - 1;
   }
-  out[k] = lo < n ? lo : n - 1;
 }
 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_collapse(typename Cplx<R>::V* __restrict__ s, uint64_t n,
                                                      uint64_t mask, uint64_t val) {
-  uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= n) return;
-  if ((i & mask) != val) s[i] = cmk<typename Cplx<R>::V>(0, 0);
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock)
+    if ((i & mask) != val) s[i] = cmk<typename Cplx<R>::V>(0, 0);
 }
 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_reset(typename Cplx<R>::V* __restrict__ s, LockedBits lk,
                                                   uint64_t ngroups, uint64_t from, uint64_t sm) {
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= ngroups) return;
-  const uint64_t b = insert_zeros(g, lk.pos, lk.npos) | from;
-  typename Cplx<R>::V a = s[b];
-  s[b ^ sm] = a;
-  s[b] = cmk<typename Cplx<R>::V>(0, 0);
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    const uint64_t b = insert_zeros(g, lk.pos, lk.npos) | from;
+    typename Cplx<R>::V a = s[b];
+    s[b ^ sm] = a;
+    s[b] = cmk<typename Cplx<R>::V>(0, 0);
+  }
 }
 
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_remove(const typename Cplx<R>::V* __restrict__ s,
                                                    typename Cplx<R>::V* __restrict__ dst, uint64_t nnew,
                                                    LockedBits lk, uint64_t fixed) {
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= nnew) return;
-  typename Cplx<R>::V a = s[insert_zeros(g, lk.pos, lk.npos) | fixed];
-  const double ns = double(a.x) * double(a.x) + double(a.y) * double(a.y);
-  dst[g] = ns < 1e-30 ? cmk<typename Cplx<R>::V>(0, 0) : a;
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < nnew; g += uint64_t(gridDim.x) * kBlock) {
+    typename Cplx<R>::V a = s[insert_zeros(g, lk.pos, lk.npos) | fixed];
+    const double ns = double(a.x) * double(a.x) + double(a.y) * double(a.y);
+    dst[g] = ns < 1e-30 ? cmk<typename Cplx<R>::V>(0, 0) : a;
+  }
 }
 
-inline unsigned nblk(uint64_t n) { return unsigned((n + kBlock - 1) / kBlock); }
+inline unsigned nblk(uint64_t n) { return capped_grid(n, kBlock); }
 
 }  // namespace
 

@@ -5,6 +5,8 @@
 #include <vector>
 
 #include "common.h"
+#include "host.h"
+#include "kernels.h"
 
 namespace yh {
 
@@ -41,6 +43,8 @@ struct Workspace {
   DevBuf shots;      // caller-supplied uniforms
   DevBuf shots_out;  // sampled outcomes
   DevBuf tiles;      // tile-pass groups and micro gates
+  DevBuf acc;        // density matrix: sum of Kraus branches (copied into rho)
+  DevBuf branch;     // density matrix: one Kraus branch
   PinBuf pin;
   PinBuf bpin;       // block matrices staged for async copies (yh_apply_fused)
   // launch timing (measurement hooks): events around each launch while enabled
@@ -68,6 +72,11 @@ struct Workspace {
 };
 
 int ensure_init();
+
+// Terms of an operator polynomial in device form. lsb_sites: sites are bit
+// positions (expect_dm); otherwise loc l is bit nbits-1-l.
+int lower_terms(int nbits, const std::vector<Term>& terms, bool lsb_sites, std::vector<TermDev>& out,
+                std::vector<cd>& coeffs);
 
 template <class R>
 int apply_gates(Workspace& w, void* state, int nbits, const yh_op* ops, size_t nops);

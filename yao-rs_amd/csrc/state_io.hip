@@ -66,10 +66,12 @@ struct JsonCursor {
     }
     return true;
   }
-  // skips any JSON value (for fields the header does not define)
-  bool skip() {
+  // skips any JSON value (for fields the header does not define); values nested
+  // deeper than kMaxDepth are rejected, as serde_json's recursion limit (128) does
+  static constexpr int kMaxDepth = 128;
+  bool skip(int depth = 0) {
     ws();
-    if (i >= s.size()) return false;
+    if (i >= s.size() || depth >= kMaxDepth) return false;
     const char c = s[i];
     if (c == '"') {
       std::string t;
@@ -84,7 +86,7 @@ struct JsonCursor {
           std::string k;
           if (!string(k) || !eat(':')) return false;
         }
-        if (!skip()) return false;
+        if (!skip(depth + 1)) return false;
       } while (eat(','));
       return eat(close);
     }

@@ -29,24 +29,26 @@ __global__ __launch_bounds__(kBlock) void k_1q(typename Cplx<R>::V* __restrict__
   using V = typename Cplx<R>::V;
   const uint64_t lowmask = (uint64_t(1) << bit) - 1;
   const uint64_t step = uint64_t(1) << bit;
-  uint64_t g0 = uint64_t(blockIdx.x) * (kBlock * kPairs) + threadIdx.x;
-  V w[kPairs], v[kPairs];
-  uint64_t i0[kPairs];
+  const uint64_t stride = uint64_t(gridDim.x) * (kBlock * kPairs);
+  for (uint64_t g0 = uint64_t(blockIdx.x) * (kBlock * kPairs) + threadIdx.x; g0 < npairs; g0 += stride) {
+    V w[kPairs], v[kPairs];
+    uint64_t i0[kPairs];
 #pragma unroll
-  for (int u = 0; u < kPairs; ++u) {
-    uint64_t g = g0 + uint64_t(u) * kBlock;
-    i0[u] = ((g >> bit) << (bit + 1)) | (g & lowmask);
-    if (g < npairs) {
-      w[u] = s[i0[u]];
-      v[u] = s[i0[u] + step];
+    for (int u = 0; u < kPairs; ++u) {
+      uint64_t g = g0 + uint64_t(u) * kBlock;
+      i0[u] = ((g >> bit) << (bit + 1)) | (g & lowmask);
+      if (g < npairs) {
+        w[u] = s[i0[u]];
+        v[u] = s[i0[u] + step];
+      }
     }
-  }
 #pragma unroll
-  for (int u = 0; u < kPairs; ++u) {
-    uint64_t g = g0 + uint64_t(u) * kBlock;
-    if (g < npairs) {
-      s[i0[u]] = cmul2(m.a, w[u], m.b, v[u]);
-      s[i0[u] + step] = cmul2(m.c, w[u], m.d, v[u]);
+    for (int u = 0; u < kPairs; ++u) {
+      uint64_t g = g0 + uint64_t(u) * kBlock;
+      if (g < npairs) {
+        s[i0[u]] = cmul2(m.a, w[u], m.b, v[u]);
+        s[i0[u] + step] = cmul2(m.c, w[u], m.d, v[u]);
+      }
     }
   }
 }
@@ -62,18 +64,18 @@ __global__ __launch_bounds__(kBlock) void k_dense(typename Cplx<R>::V* __restric
                                                   const typename Cplx<R>::V* __restrict__ mat) {
   using V = typename Cplx<R>::V;
   constexpr int D = 1 << NT;
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= ga.ngroups) return;
-  uint64_t base = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
-  V in[D];
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < ga.ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    uint64_t base = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
+    V in[D];
 #pragma unroll
-  for (int j = 0; j < D; ++j) in[j] = s[base + ga.offs[j]];
+    for (int j = 0; j < D; ++j) in[j] = s[base + ga.offs[j]];
 #pragma unroll
-  for (int r = 0; r < D; ++r) {
-    V acc = cmul(mat[r * D], in[0]);
+    for (int r = 0; r < D; ++r) {
+      V acc = cmul(mat[r * D], in[0]);
 #pragma unroll
-    for (int c = 1; c < D; ++c) acc = cadd(acc, cmul(mat[r * D + c], in[c]));
-    s[base + ga.offs[r]] = acc;
+      for (int c = 1; c < D; ++c) acc = cadd(acc, cmul(mat[r * D + c], in[c]));
+      s[base + ga.offs[r]] = acc;
+    }
   }
 }
 
@@ -84,12 +86,12 @@ __global__ __launch_bounds__(kBlock) void k_dense(typename Cplx<R>::V* __restric
 template <class R, int NT>
 __global__ __launch_bounds__(kBlock) void k_diag(typename Cplx<R>::V* __restrict__ s, GroupArgs ga,
                                                  DiagArgs<R> d) {
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= ga.ngroups) return;
-  uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
-  int j = int((i >> d.tbit[0]) & 1);
-  if (NT == 2) j |= int((i >> d.tbit[1]) & 1) << 1;
-  s[i] = cmul(s[i], d.d[j]);
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < ga.ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
+    int j = int((i >> d.tbit[0]) & 1);
+    if (NT == 2) j |= int((i >> d.tbit[1]) & 1) << 1;
+    s[i] = cmul(s[i], d.d[j]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -99,12 +101,12 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_xflip(typename Cplx<R>::V* __restrict__ s, GroupArgs ga,
                                                   uint64_t tmask) {
   using V = typename Cplx<R>::V;
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= ga.ngroups) return;
-  uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
-  V a = s[i], b = s[i | tmask];
-  s[i] = b;
-  s[i | tmask] = a;
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < ga.ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
+    V a = s[i], b = s[i | tmask];
+    s[i] = b;
+    s[i | tmask] = a;
+  }
 }
 
 // SWAP: exchange the amplitudes with (b0, b1) = (0, 1) and (1, 0).
@@ -112,13 +114,13 @@ template <class R>
 __global__ __launch_bounds__(kBlock) void k_swap(typename Cplx<R>::V* __restrict__ s, GroupArgs ga,
                                                  uint64_t m0, uint64_t m1) {
   using V = typename Cplx<R>::V;
-  uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (g >= ga.ngroups) return;
-  uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set | m1;
-  uint64_t j = (i ^ m1) | m0;
-  V a = s[i], b = s[j];
-  s[i] = b;
-  s[j] = a;
+  for (uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x; g < ga.ngroups; g += uint64_t(gridDim.x) * kBlock) {
+    uint64_t i = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set | m1;
+    uint64_t j = (i ^ m1) | m0;
+    V a = s[i], b = s[j];
+    s[i] = b;
+    s[j] = a;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -133,19 +135,17 @@ __global__ __launch_bounds__(kBlock) void k_dense_oop(typename Cplx<R>::V* __res
                                                       const uint64_t* __restrict__ offs) {
   using V = typename Cplx<R>::V;
   const uint64_t D = uint64_t(1) << nt;
-  uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
-  if (t >= ga.ngroups * D) return;
-  uint64_t g = t >> nt, r = t & (D - 1);
-  uint64_t base = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
-  V acc = cmk<V>(0, 0);
-  for (uint64_t c = 0; c < D; ++c) acc = cadd(acc, cmul(mat[r * D + c], src[base + offs[c]]));
-  dst[base + offs[r]] = acc;
+  for (uint64_t t = uint64_t(blockIdx.x) * kBlock + threadIdx.x; t < ga.ngroups * D; t += uint64_t(gridDim.x) * kBlock) {
+    uint64_t g = t >> nt, r = t & (D - 1);
+    uint64_t base = insert_zeros(g, ga.lk.pos, ga.lk.npos) | ga.ctrl_set;
+    V acc = cmk<V>(0, 0);
+    for (uint64_t c = 0; c < D; ++c) acc = cadd(acc, cmul(mat[r * D + c], src[base + offs[c]]));
+    dst[base + offs[r]] = acc;
+  }
 }
 
 // ============================== launchers ==================================
-static inline unsigned grid_for(uint64_t work, uint64_t per_block) {
-  return unsigned((work + per_block - 1) / per_block);
-}
+static inline unsigned grid_for(uint64_t work, uint64_t per_block) { return capped_grid(work, per_block); }
 
 template <class R>
 int launch_1q(typename Cplx<R>::V* s, int nbits, int bit, const Mat2<R>& m, hipStream_t st) {

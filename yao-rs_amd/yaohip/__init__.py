@@ -9,7 +9,7 @@ from .circuit import (Annotation, Circuit, CircuitError, Gate, NoiseChannel, Op,
                       put, tfim_hamiltonian)
 from .circuit_json import CircuitJsonError, circuit_from_json, circuit_to_json
 from .register import (ArrayReg, DensityMatrix, PostProcess, apply, apply_inplace, channel_kraus,
-                       expect_arrayreg, expect_dm, expect_grad, gate_matrix, measure_with_postprocess, probs,
+                       density_matrix_from_reg, expect_arrayreg, expect_dm, expect_grad, gate_matrix, measure_with_postprocess, probs,
                        read_state, read_state_header, sample, write_state)
 
 YhError = _abi.YhError
@@ -17,7 +17,7 @@ YhError = _abi.YhError
 __all__ = [
     "Annotation", "ArrayReg", "Circuit", "CircuitError", "CircuitJsonError", "DensityMatrix", "Gate", "NoiseChannel", "Op",
     "OperatorPolynomial", "OperatorString", "PositionedChannel", "PositionedGate", "PostProcess", "YhError",
-    "apply", "apply_inplace", "channel", "channel_kraus", "circuit_from_json", "circuit_to_json", "control", "expect_arrayreg", "expect_dm", "expect_grad",
+    "apply", "apply_inplace", "channel", "channel_kraus", "circuit_from_json", "density_matrix_from_reg", "circuit_to_json", "control", "expect_arrayreg", "expect_dm", "expect_grad",
     "gate_matrix", "label", "lower", "measure_with_postprocess", "probs", "put", "read_state", "read_state_header",
     "sample", "tfim_hamiltonian", "write_state",
 ]

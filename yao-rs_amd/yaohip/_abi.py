@@ -126,6 +126,11 @@ SIGNATURES = [
     ("yh_reg_normalize", c_int, [c_void_p]),
     ("yh_reg_fidelity", c_int, [c_void_p, c_void_p, P(c_double)]),
     ("yh_reg_device_ptr", c_int, [c_void_p, P(c_void_p)]),
+    ("yh_reg_product_state", c_int, [c_int, c_int, ctypes.c_uint64, P(c_void_p)]),
+    ("yh_reg_uniform_state", c_int, [c_int, c_int, P(c_void_p)]),
+    ("yh_reg_ghz_state", c_int, [c_int, c_int, P(c_void_p)]),
+    ("yh_reg_deterministic_state", c_int, [c_int, c_int, P(c_void_p)]),
+    ("yh_reg_rand_state", c_int, [c_int, c_int, P(c_double), c_size_t, P(c_void_p)]),
     ("yh_state_write", c_int, [c_void_p, ctypes.c_char_p]),
     ("yh_state_read", c_int, [ctypes.c_char_p, c_int, P(c_void_p)]),
     ("yh_state_read_header", c_int, [ctypes.c_char_p, P(c_int)]),
@@ -150,6 +155,8 @@ SIGNATURES = [
                            c_double, P(ctypes.c_int32)]),
     ("yh_dm_create", c_int, [c_int, P(c_void_p)]),
     ("yh_dm_from_reg", c_int, [c_void_p, P(c_void_p)]),
+    ("yh_dm_mixed", c_int, [P(c_double), P(c_void_p), c_size_t, P(c_void_p)]),
+    ("yh_dm_from_reg_locs", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_void_p)]),
     ("yh_dm_destroy", c_int, [c_void_p]),
     ("yh_dm_nbits", c_int, [c_void_p, P(c_int)]),
     ("yh_dm_upload", c_int, [c_void_p, P(c_double), c_size_t]),

@@ -49,17 +49,42 @@ class ArrayReg:
         return r
 
     @staticmethod
+    def _made(fn, dtype, *args):
+        h = ctypes.c_void_p()
+        A.check(fn(*args[:1], A.C128 if dtype == "c128" else A.C64, *args[1:], ctypes.byref(h)))
+        return ArrayReg(h, dtype)
+
+    @staticmethod
     def uniform_state(nbits, dtype="c128"):
         """uniform_state (register.rs:47-54)."""
-        amp = 1.0 / np.sqrt(float(1 << nbits))
-        return ArrayReg.from_vec(nbits, np.full(1 << nbits, amp, dtype=np.complex128), dtype)
+        return ArrayReg._made(_lib().yh_reg_uniform_state, dtype, nbits)
 
     @staticmethod
     def ghz_state(nbits, dtype="c128"):
         """ghz_state (register.rs:56-63)."""
-        s = np.zeros(1 << nbits, dtype=np.complex128)
-        s[0] = s[-1] = 1.0 / np.sqrt(2.0)
-        return ArrayReg.from_vec(nbits, s, dtype)
+        return ArrayReg._made(_lib().yh_reg_ghz_state, dtype, nbits)
+
+    @staticmethod
+    def product_state(nbits, bits, dtype="c128"):
+        """product_state (register.rs:28-33): BitStr::onehot of `bits`, an integer
+        basis index or a bit string written most significant bit first."""
+        index = int(bits, 2) if isinstance(bits, str) else int(bits)
+        return ArrayReg._made(_lib().yh_reg_product_state, dtype, nbits, ctypes.c_uint64(index))
+
+    @staticmethod
+    def deterministic_state(nbits, dtype="c128"):
+        """deterministic_state (register.rs:71-84)."""
+        return ArrayReg._made(_lib().yh_reg_deterministic_state, dtype, nbits)
+
+    @staticmethod
+    def rand_state(nbits, rng, dtype="c128"):
+        """rand_state (register.rs:35-45). `rng` is a callable returning an array of k
+        f64 uniforms in [0, 1) for rng(k) (the caller owns the stream, as `&mut impl
+        Rng`), or a numpy Generator; 2 uniforms per amplitude, (re, im) in order."""
+        k = 2 << nbits
+        u = rng.random(k) if hasattr(rng, "random") else rng(k)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return ArrayReg._made(_lib().yh_reg_rand_state, dtype, nbits, A.dptr(u), ctypes.c_size_t(k))
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -124,16 +149,20 @@ def read_state_header(path):
     return n.value
 
 
-def apply_inplace(circuit, reg, fused=0):
-    """apply_inplace (apply.rs:123-147). fused=k>0 packs gate runs on up to k qubits."""
+def apply_inplace(circuit, reg, fused=None):
+    """apply_inplace (apply.rs:123-147). fused=None (default) or 5: gates fused into
+    MFMA blocks of up to 5 qubits (yh_apply_fused, the same default as the C ABI's
+    Rust binding); 1..4: LDS tile passes; 0: one launch per gate (yh_apply)."""
     ops = circuit.ops
+    if fused is None:
+        fused = 5
     if fused:
         A.check(A.lib().yh_apply_fused(reg._h, circuit.nbits, ops.arr, ops.n, int(fused)))
     else:
         A.check(A.lib().yh_apply(reg._h, circuit.nbits, ops.arr, ops.n))
 
 
-def apply(circuit, reg, fused=0):
+def apply(circuit, reg, fused=None):
     """apply (apply.rs:150-154): returns a new register."""
     out = reg.clone()
     apply_inplace(circuit, out, fused)
@@ -232,6 +261,17 @@ class DensityMatrix:
         return DensityMatrix(h)
 
     @staticmethod
+    def mixed(weights, regs):
+        """mixed (density_matrix.rs:37-61): sum_j w_j |psi_j><psi_j|."""
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        if len(w) != len(regs):
+            raise ValueError("weights and registers differ in length")
+        hs = (ctypes.c_void_p * max(1, len(regs)))(*[r._h for r in regs])
+        h = ctypes.c_void_p()
+        A.check(A.lib().yh_dm_mixed(A.dptr(w), hs, len(regs), ctypes.byref(h)))
+        return DensityMatrix(h)
+
+    @staticmethod
     def from_matrix(nbits, rho):
         d = DensityMatrix.zero_state(nbits)
         buf = A.interleave(np.asarray(rho).reshape(-1))
@@ -291,6 +331,15 @@ class DensityMatrix:
         v = ctypes.c_double()
         A.check(A.lib().yh_dm_entropy(self._h, ctypes.byref(v)))
         return v.value
+
+
+def density_matrix_from_reg(reg, locs):
+    """density_matrix_from_reg (density_matrix.rs:266-272): the reduced density matrix
+    on `locs` (bit positions, as partial_tr reads them), without the full matrix."""
+    la = np.array(list(locs) or [0], dtype=np.int32)
+    h = ctypes.c_void_p()
+    A.check(A.lib().yh_dm_from_reg_locs(reg._h, A.i32ptr(la), len(locs), ctypes.byref(h)))
+    return DensityMatrix(h)
 
 
 def expect_dm(dm, op: OperatorPolynomial):
