@@ -222,6 +222,20 @@ int yh_expect(const yh_reg *reg, size_t nterms, const double *coeffs,
               const int32_t *term_len, const int32_t *sites, const int32_t *ops,
               double out[2]);                                       /* expect_arrayreg :7 */
 
+/* No reference counterpart: expect_arrayreg (expect.rs:7-11) for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). `amps` is a caller-owned device state of 2^nbits
+ * amplitudes (interleaved re, im; double for YH_C128, float for YH_C64) on the
+ * current device; terms as for yh_expect; `stream` a hipStream_t (NULL: the null
+ * stream), synchronised before return. The reduction partials, results and at most
+ * 256 terms live in the library's static device array (about 760 KiB per device,
+ * held until the process ends); more terms return YH_ERR_INVALID_ARG. No device or
+ * pinned memory is allocated. Every call on one GPU shares that array: calls that
+ * overlap on one GPU race. Results equal yh_expect's (same kernels, same order). */
+int yh_expect_noalloc(const void *amps, int nbits, int dtype, size_t nterms, const double *coeffs,
+                      const int32_t *term_len, const int32_t *sites, const int32_t *ops, void *stream,
+                      double out[2]);
+
 /* ---- measurement (src/measure.rs) ---- */
 /* probs (measure.rs:82-104): locs == NULL -> full distribution (2^n values),
  * else marginal over locs (2^nlocs values, output bit nlocs-1-i <- locs[i]). */

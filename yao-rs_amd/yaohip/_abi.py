@@ -148,6 +148,8 @@ SIGNATURES = [
     ("yh_gate_matrix", c_int, [P(YhOp), P(c_double), P(c_int)]),
     ("yh_expect", c_int, [c_void_p, c_size_t, P(c_double), P(ctypes.c_int32),
                           P(ctypes.c_int32), P(ctypes.c_int32), P(c_double)]),
+    ("yh_expect_noalloc", c_int, [c_void_p, c_int, c_int, c_size_t, P(c_double), P(ctypes.c_int32),
+                                  P(ctypes.c_int32), P(ctypes.c_int32), c_void_p, P(c_double)]),
     ("yh_probs", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_double)]),
     ("yh_sample", c_int, [c_void_p, P(ctypes.c_int32), c_int, P(c_double), c_size_t,
                           P(ctypes.c_uint64)]),
