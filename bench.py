@@ -391,15 +391,15 @@ def bench_vqe(log, count=4096, plain_sample=256):
 
 
 def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=65536, keep_frac=0.01,
-                    schedule=((40, 1e-3), (10, 3e-4)), batch=4096, mae_sample=4096, plain_sample=64):
+                    schedule=((40, 1e-3), (10, 3e-4)), batch=4096, plain_sample=64):
     """Learned surrogate: 288 angles (16 qubits, depth 6, Rz-Ry-Rz per qubit per
     layer, CZ chain; angles uniform in [-pi/4, pi/4]) -> TFIM energy (J = h = 1).
     MLP 576 trig features -> 4 x 256 GELU -> 1, BF16 MFMA with FP32 accumulation,
     trained by the library's kernels on 2^20 pairs from the batched simulator
     (complex64); screens 65536 candidates and re-evaluates the best 1% exactly
-    (batched complex128). Reports the surrogate's MAE against exact energies and
-    candidate evaluations/s of the screened pipeline against the plain per-circuit
-    GPU path and the batched exact path on the same ansatz."""
+    (batched complex128). Every candidate is also evaluated exactly (the judge):
+    reports the surrogate's MAE, recall of the true best 1 % and the regret of the
+    best energy it finds, with the screened and exhaustive times."""
     from yaohip.surrogate import SurrogateScreen
     from yaohip.vqe import fixed_instances, plain_energies
     s = SurrogateScreen(n=n, depth=depth, apq=apq, width=256, hidden=4, max_batch=4096, seed=0)
@@ -407,7 +407,7 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
            "network": "576 (cos, sin) features -> 4 x 256 GELU -> 1; bf16 weights/activations, fp32 accum",
            "train_pairs": train_pairs, "candidates": candidates, "keep_frac": keep_frac}
     train = s.sample_angles(train_pairs, seed=101)
-    s.exact_energies(train[:4096], dtype="c64")  # autotune + graph capture
+    s.exact_energies(s.sample_angles(s.chunk, seed=100), dtype="c64")  # same shape list: autotune, capture
     t0 = time.perf_counter()
     e = s.exact_energies(train, dtype="c64")
     tg = time.perf_counter() - t0
@@ -433,26 +433,35 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
     _, _, _, tm0 = s.screen(s.sample_angles(candidates, seed=201), keep_frac=keep_frac, dtype="c128")
     idx, exact, pred, tm = s.screen(cand, keep_frac=keep_frac, dtype="c128")
     t_screen = tm["predict_s"] + tm["reeval_s"]
-    res["screen"] = {"predict_s": round(tm["predict_s"], 4), "reeval_s": round(tm["reeval_s"], 4),
-                     "reevaluated": int(idx.size), "first_call_reeval_s": round(tm0["reeval_s"], 4),
-                     "predictions_per_s": round(candidates / tm["predict_s"], 1),
-                     "screened_candidates_per_s": round(candidates / t_screen, 1)}
-    # the surrogate's error, judged by the exact simulator on a sample of the candidates
-    rng = np.random.default_rng(303)
-    samp = rng.choice(candidates, size=mae_sample, replace=False)
-    s.exact_energies(s.sample_angles(mae_sample, seed=304), dtype="c128")  # same shape list: autotune, graph
+    # the judge: every candidate evaluated exactly by the batched complex128 simulator
+    s.exact_energies(s.sample_angles(candidates, seed=204), dtype="c128")  # same shape list: tune, capture
     t0 = time.perf_counter()
-    ex = s.exact_energies(cand[samp], dtype="c128")
-    tb = time.perf_counter() - t0
-    rand = samp[:idx.size]
-    res["quality"] = {"mae": round(float(np.mean(np.abs(pred[samp] - ex))), 5),
-                      "mae_sample": mae_sample, "energy_std": round(float(ex.std()), 5),
-                      "top_mean_exact": round(float(exact.mean()), 5), "top_min_exact": round(float(exact.min()), 5),
-                      "random_same_count_mean_exact": round(float(ex[:idx.size].mean()), 5),
-                      "random_same_count_min_exact": round(float(ex[:idx.size].min()), 5),
-                      "sample_mean_exact": round(float(ex.mean()), 5)}
-    del rand
-    res["batched_c128_evals_per_s"] = round(mae_sample / tb, 1)
+    ex_all = s.exact_energies(cand, dtype="c128")
+    t_exh = time.perf_counter() - t0
+    k = int(idx.size)
+    true_top = np.argsort(ex_all, kind="stable")[:k]
+    recall = len(np.intersect1d(idx, true_top)) / k
+    regret = float(exact.min() - ex_all.min())
+    rng = np.random.default_rng(303)
+    rand = rng.choice(candidates, size=k, replace=False)
+    res["screen"] = {"predict_s": round(tm["predict_s"], 4), "reeval_s": round(tm["reeval_s"], 4),
+                     "reevaluated": k, "first_call_reeval_s": round(tm0["reeval_s"], 4),
+                     "predictions_per_s": round(candidates / tm["predict_s"], 1),
+                     "exact_evals_per_s": round(k / t_screen, 1),
+                     "seconds": round(t_screen, 4)}
+    res["exhaustive"] = {"exact_evals": candidates, "seconds": round(t_exh, 4),
+                         "exact_evals_per_s": round(candidates / t_exh, 1),
+                         "best_exact": round(float(ex_all.min()), 6)}
+    res["quality"] = {"mae_all_candidates": round(float(np.mean(np.abs(pred - ex_all))), 5),
+                      "energy_std": round(float(ex_all.std()), 5),
+                      "recall_at_1pct": round(recall, 4),
+                      "best_found_regret": round(regret, 6),
+                      "best_found_exact": round(float(exact.min()), 6),
+                      "top_mean_exact": round(float(exact.mean()), 5),
+                      "random_same_count_mean_exact": round(float(ex_all[rand].mean()), 5),
+                      "random_same_count_best_exact": round(float(ex_all[rand].min()), 5),
+                      "pool_mean_exact": round(float(ex_all.mean()), 5)}
+    res["time_to_best_found_vs_exhaustive"] = round(t_exh / t_screen, 2)
     # the plain per-circuit path on the same ansatz
     sub = fixed_instances(cand[:plain_sample], n, depth, apq)
     plain_energies(fixed_instances(cand[:2], n, depth, apq))
@@ -460,12 +469,12 @@ def bench_surrogate(log, n=16, depth=6, apq=3, train_pairs=1 << 20, candidates=6
     plain_energies(sub, dtype="c128")
     tp = time.perf_counter() - t0
     res["plain_c128_evals_per_s"] = round(plain_sample / tp, 1)
-    res["screened_gain_vs_plain"] = round(candidates / t_screen / res["plain_c128_evals_per_s"], 1)
-    res["screened_gain_vs_batched_exact"] = round(candidates / t_screen / res["batched_c128_evals_per_s"], 2)
-    res["note"] = ("screened candidates/s count surrogate predictions plus exact re-evaluation of the kept "
-                   "fraction, with the simulator already tuned for the kept count; data generation, training "
-                   "and the simulator's first call (autotune + graph capture, first_call_reeval_s) are reported "
-                   "separately (one-off per ansatz)")
+    res["batched_vs_plain"] = round(res["exhaustive"]["exact_evals_per_s"] / res["plain_c128_evals_per_s"], 1)
+    res["note"] = ("only exact simulator energies count as evaluations: the screened pipeline "
+                   "evaluates the surrogate's best 1% exactly (exact_evals_per_s counts those, over "
+                   "prediction plus re-evaluation time); recall_at_1pct and best_found_regret compare its "
+                   "choice with exhaustive exact evaluation of the same pool; data generation, training "
+                   "and the simulator's first call (autotune + graph capture) are reported separately")
     log(f"surrogate: {res}")
     return res
 

@@ -206,6 +206,31 @@ def test_sample_bit_exact_with_same_uniforms(gpu):
         assert np.array_equal(got, want), locs
 
 
+@pytest.mark.parametrize("n", [1, 3, 13, 22])
+def test_sample_bit_exact_cdf_chunks(gpu, n):
+    """sample_from_probs (measure.rs:140-150) over distributions of 2 .. 2^22 bins:
+    the device's sequential cumulative sum is folded in LDS chunks of 4096 with
+    ragged ends; every shot, including uniforms exactly at partial sums at chunk
+    edges, matches the oracle's sequential loop."""
+    psi = seeded_state(n, 60 + n)
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    p = O.probs(n, psi)
+    cs = np.cumsum(p)  # numpy's cumsum is the same sequential left fold:
+    acc, seq = 0.0, []
+    for x in p[:9000]:
+        acc += x
+        seq.append(acc)
+    assert np.array_equal(np.array(seq), cs[:len(seq)])
+    edges = [i for i in (0, 1, 4094, 4095, 4096, 4097, 8191, 8192, len(p) - 2, len(p) - 1) if 0 <= i < len(p)]
+    u = np.concatenate([np.random.default_rng(n).random(3000), cs[edges], np.nextafter(cs[edges], 0.0)])
+    got = gpu.sample(reg, None, u)
+    want = np.searchsorted(cs, u, side="right")
+    want[want >= len(p)] = len(p) - 1
+    assert np.array_equal(got, want)
+    if n <= 13:
+        assert np.array_equal(got, O.sample(n, psi, None, u))
+
+
 @pytest.mark.parametrize("post", [0, 1, 2])
 def test_measure_postprocess_vs_oracle(gpu, post):
     n = 8

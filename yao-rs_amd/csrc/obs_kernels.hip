@@ -237,25 +237,71 @@ __global__ __launch_bounds__(kBlock) void k_dm_diag_probs(const double2* __restr
     out[i] = fmax(rho[i * dim + i].x, 0.0);
 }
 
-// Sequential cumulative sum in the reference's order (measure.rs:140-150).
-__global__ void k_cdf(const double* __restrict__ p, uint64_t n, double* __restrict__ cdf) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// Sequential cumulative sum in the reference's order (measure.rs:140-150):
+// cdf[i] = (((p[0] + p[1]) + p[2]) + ... + p[i]), one FP64 rounding per addition.
+// The additions form one dependent chain, folded by one lane (thread 0) out of LDS;
+// waves 1..3 stream the distribution through two LDS chunks around it (store the
+// previous chunk's sums to cdf, load the next chunk's probabilities), so the chain
+// never waits for HBM. Thread 0 reads its chunk eight values ahead of its writes.
+constexpr int kCdfChunk = 4096;
+constexpr int kCdfThreads = 256;
+__global__ __launch_bounds__(kCdfThreads) void k_cdf(const double* __restrict__ p, uint64_t n,
+                                                     double* __restrict__ cdf) {
+  __shared__ double buf[2][kCdfChunk];
+  const int tid = threadIdx.x;
+  const int mover = tid - 64, nmov = kCdfThreads - 64;
+  const uint64_t nch = (n + kCdfChunk - 1) / kCdfChunk;
+  auto load = [&](uint64_t c) {
+    const uint64_t base = c * kCdfChunk;
+    for (int j = mover; j < kCdfChunk; j += nmov) buf[c & 1][j] = base + j < n ? p[base + j] : 0.0;
+  };
+  auto store = [&](uint64_t c) {
+    const uint64_t base = c * kCdfChunk;
+    for (int j = mover; j < kCdfChunk; j += nmov)
+      if (base + j < n) cdf[base + j] = buf[c & 1][j];
+  };
+  if (mover >= 0) load(0);
+  __syncthreads();
   double cs = 0.0;
-  uint64_t i = 0;
-  for (; i + 8 <= n; i += 8) {
-    double x[8];
+  for (uint64_t c = 0; c < nch; ++c) {
+    if (mover >= 0) {
+      // the same thread stores, then refills, the same positions of the other chunk
+      if (c >= 1) store(c - 1);
+      if (c + 1 < nch) load(c + 1);
+    } else if (tid == 0) {
+      double* b = buf[c & 1];
+      const int m = c + 1 < nch ? kCdfChunk : int(n - c * kCdfChunk);
+      int j = 0;
+      double x[8];
+      if (m >= 8) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) x[k] = p[i + k];
+        for (int k = 0; k < 8; ++k) x[k] = b[k];
+      }
+      for (; j + 8 <= m; j += 8) {
+        double y[8];
+        const bool more = j + 16 <= m;
+        if (more) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      cs = __dadd_rn(cs, x[k]);
-      cdf[i + k] = cs;
+          for (int k = 0; k < 8; ++k) y[k] = b[j + 8 + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          cs = __dadd_rn(cs, x[k]);
+          b[j + k] = cs;
+        }
+        if (more) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) x[k] = y[k];
+        }
+      }
+      for (; j < m; ++j) {
+        cs = __dadd_rn(cs, b[j]);
+        b[j] = cs;
+      }
     }
+    __syncthreads();
   }
-  for (; i < n; ++i) {
-    cs = __dadd_rn(cs, p[i]);
-    cdf[i] = cs;
-  }
+  if (mover >= 0) store(nch - 1);
 }
 
 // First i with r < cdf[i] (cdf is non-decreasing), else n-1.
@@ -403,7 +449,7 @@ int dm_probs_dev(const double2* rho, int nbits, const int32_t* locbits_dev, cons
 }
 
 int launch_cdf(const double* p, uint64_t n, double* cdf, hipStream_t st) {
-  hipLaunchKernelGGL(k_cdf, dim3(1), dim3(64), 0, st, p, n, cdf);
+  hipLaunchKernelGGL(k_cdf, dim3(1), dim3(kCdfThreads), 0, st, p, n, cdf);
   return YH_OK;
 }
 
