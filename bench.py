@@ -56,36 +56,43 @@ def gate_bytes(e, nbits, amp_bytes=16):
     return 2 * amp_bytes * (1 << (nbits - len(e.control_locs)))
 
 
-def cpu_baseline(els, nbits):
-    """Oracle (single-threaded C++ restatement of the reference) on a bounded sample:
-    2 gates of each class at full size, extrapolated to the circuit's gate mix."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(els, nbits, locs=(0, 9, 18, 27), pairs=((8, 9), (18, 19))):
+    """Oracle (single-threaded C++ restatement of the reference, -O3) on a bounded
+    sample of the circuit: layer 0's rotations on `locs` and its CNOTs on `pairs`
+    (12 + 2 gates, the 6 : 1 rotation-to-CNOT mix of every layer, targets spread
+    over high and low bits), applied to the full-size state and timed whole."""
     from oracle import oracle as O
     from yaohip.circuit import lower
+    layer0 = els[:3 * nbits + len(range(0, nbits - 1, 2))]
+    sample = [e for e in layer0 if (not e.control_locs and e.target_locs[0] in locs) or
+              (e.control_locs and (e.control_locs[0], e.target_locs[0]) in pairs)]
     buf = np.zeros(2 << nbits)
     buf[0] = 1.0
     buf[1:] = 0.0  # touch every page before timing
-    classes = {}
-    for e in els:
-        k = "cnot" if e.control_locs else ("rz" if e.gate.kind == 10 else "rot")
-        classes.setdefault(k, []).append(e)
-    per = {}
-    t_all = 0.0
-    for k, lst in classes.items():
-        sample = lst[:2]
-        ol = lower(sample)
-        t0 = time.perf_counter()
-        st = O.lib().orc_apply(nbits, nbits, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ol.arr, ol.n)
-        dt = time.perf_counter() - t0
-        assert st == 0
-        per[k] = dt / len(sample)
-        t_all += dt
-    est = sum(per[k] * len(v) for k, v in classes.items())
-    return {"value": len(els) / est, "unit": "gate-apps/s", "cores": 1, "kind": "port",
-            "sample": (f"oracle (C++ restatement of yao-rs instruct/dispatch, -O2, 1 thread) on the 28-qubit "
-                       f"state: 2 gates of each class (Rx/Ry dense, Rz diagonal, CNOT) timed "
-                       f"({t_all:.1f} s), per-class times weighted by the circuit's gate mix "
-                       f"({', '.join(f'{k}={len(v)}' for k, v in classes.items())})"),
-            "per_gate_s": per}
+    prev = O.set_threads(1)
+    ol = lower(sample)
+    t0 = time.perf_counter()
+    st = O.lib().orc_apply(nbits, nbits, buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ol.arr, ol.n)
+    dt = time.perf_counter() - t0
+    O.set_threads(prev)
+    assert st == 0
+    return {"value": len(sample) / dt, "unit": "gate-apps/s", "cores": 1, "kind": "port",
+            "cpu_model": cpu_model(),
+            "sample": (f"oracle (C++ restatement of yao-rs instruct/dispatch, g++ -O3 -ffp-contract=off, one "
+                       f"thread) on the full {nbits}-qubit state: layer 0's Rx, Ry, Rz on locs {list(locs)} and "
+                       f"its CNOTs on {[list(p) for p in pairs]} ({len(sample)} gates, the circuit's 6:1 "
+                       f"rotation-to-CNOT mix), timed whole: {dt:.1f} s")}
 
 
 def main():
