@@ -527,16 +527,19 @@ __global__ __launch_bounds__(256) void k_mlp_bwd(MlpArgs a) {
 
 // Weight gradients of every hidden layer in one launch: dW_l [W x K_l] =
 // dzT_l [W x Mp] * (h_{l-1}^T [K_l x Mp])^T (h_{-1}^T = xT; both operands in fragment
-// order, so each operand load of a wave is 1 KB of contiguous memory), 64 x 64 tiles,
-// the Mp rows split into S chunks whose partial products go to wpart_l + z W K_l (summed
-// in order afterwards). Workgroups are numbered so that those on one XCD (the
-// hardware deals workgroup ids to the 8 XCDs in turn) take consecutive tiles of one
-// (layer, chunk, row band), whose A rows then come from that XCD's L2.
+// order, so each operand load of a wave is 1 KB of contiguous memory). A workgroup
+// computes a 128 x 128 tile of dW over one of S chunks of the Mp rows (its partial
+// product goes to wpart_l + z W K_l, summed in order afterwards); each wave a 64 x 64
+// quarter, so that every 16-byte operand load feeds four MFMAs. Workgroups are
+// numbered so that those on one XCD (the hardware deals workgroup ids to the 8 XCDs
+// in turn) take consecutive tiles of one (layer, chunk, row band), whose A rows then
+// come from that XCD's L2.
+constexpr int kWgTile = 128;
 struct WgradJobs {
-  int nl, S, kc, W, Mp;
-  int tiles_k[kMaxLayers];      // K_l / 64 (rounded up)
+  int nl, S, kc, W, Mp, tiles_m;
+  int tiles_k[kMaxLayers];      // K_l / 128 (rounded up)
   int k[kMaxLayers];            // K_l
-  int first[kMaxLayers + 1];    // prefix of workgroups per layer: tiles_k (W / 64) S
+  int first[kMaxLayers + 1];    // prefix of workgroups per layer: tiles_k tiles_m S
   const uint16_t* A[kMaxLayers];
   const uint16_t* B[kMaxLayers];
   float* part[kMaxLayers];
@@ -548,76 +551,86 @@ __global__ __launch_bounds__(256) void k_mlp_wgrad(WgradJobs J) {
   int l = 0;
   while (id >= J.first[l + 1]) ++l;
   int q = id - J.first[l];
-  const int tk = J.tiles_k[l], tn = J.W / 64;
+  const int tk = J.tiles_k[l], tm = J.tiles_m;
   const int x = q % tk;
   q /= tk;
-  const int y = q % tn, z = q / tn;
+  const int y = q % tm, z = q / tm;
   const int K = J.k[l], N = J.W;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m0 = y * 64 + (wave >> 1) * 32;  // rows of dW (W dimension)
-  const int n0 = x * 64 + (wave & 1) * 32;   // columns of dW (K dimension)
+  const int m0 = y * kWgTile + (wave >> 1) * 64;  // rows of dW (W dimension)
+  const int n0 = x * kWgTile + (wave & 1) * 64;   // columns of dW (K dimension)
+  if (m0 >= N || n0 >= K) return;                 // a quarter past the matrix (no barriers follow)
   const int kb = z * J.kc, ke = min(J.Mp, kb + J.kc);
-  f4 acc[2][2];
+  f4 acc[4][4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-  const uint16_t* ap[2];
-  const uint16_t* bp[2];
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+  const uint16_t* ap[4];
+  const uint16_t* bp[4];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) ap[i] = J.A[l] + size_t((m0 >> 4) + i) * 16 * J.Mp + 16 * kb + lane * 8;
+  for (int i = 0; i < 4; ++i) ap[i] = J.A[l] + size_t((m0 >> 4) + i) * 16 * J.Mp + 16 * kb + lane * 8;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) bp[j] = J.B[l] + size_t(min((n0 >> 4) + j, (K >> 4) - 1)) * 16 * J.Mp + 16 * kb + lane * 8;
-  constexpr int kChunk = 4;
+  for (int j = 0; j < 4; ++j)
+    bp[j] = J.B[l] + size_t(min((n0 >> 4) + j, (K >> 4) - 1)) * 16 * J.Mp + 16 * kb + lane * 8;
+  constexpr int kC = 2;  // steps per chunk
   struct Ops {
-    bf8 a[kChunk][2], b[kChunk][2];
+    bf8 a[kC][4], b[kC][4];
   };
   auto load = [&](Ops& o, int c) __attribute__((always_inline)) {
 #pragma unroll
-    for (int t = 0; t < kChunk; ++t) {
-      const int k0 = 512 * (c * kChunk + t);
+    for (int t = 0; t < kC; ++t) {
+      const int k0 = 512 * (c * kC + t);
 #pragma unroll
-      for (int i = 0; i < 2; ++i) o.a[t][i] = *reinterpret_cast<const bf8*>(ap[i] + k0);
+      for (int i = 0; i < 4; ++i) o.a[t][i] = *reinterpret_cast<const bf8*>(ap[i] + k0);
 #pragma unroll
-      for (int j = 0; j < 2; ++j) o.b[t][j] = *reinterpret_cast<const bf8*>(bp[j] + k0);
+      for (int j = 0; j < 4; ++j) o.b[t][j] = *reinterpret_cast<const bf8*>(bp[j] + k0);
     }
   };
   auto compute = [&](const Ops& o) __attribute__((always_inline)) {
 #pragma unroll
-    for (int t = 0; t < kChunk; ++t)
+    for (int t = 0; t < kC; ++t)
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) acc[i][j] = mfma32(o.a[t][i], o.b[t][j], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = mfma32(o.a[t][i], o.b[t][j], acc[i][j]);
   };
-  const int nsteps = (ke - kb) >> 5, nfull = nsteps / kChunk;
+  const int nsteps = (ke - kb) >> 5, nfull = nsteps / kC;
   if (nfull > 0) {
     Ops o0, o1;
     load(o0, 0);
-    for (int c = 0; c < nfull; c += 2) {
-      load(o1, min(c + 1, nfull - 1));
+    int c = 0;
+    for (; c + 2 < nfull; c += 2) {
+      load(o1, c + 1);
       compute(o0);
-      load(o0, min(c + 2, nfull - 1));
-      if (c + 1 < nfull) compute(o1);
+      load(o0, c + 2);
+      compute(o1);
+    }
+    if (c + 1 < nfull) {  // the last two chunks
+      load(o1, c + 1);
+      compute(o0);
+      compute(o1);
+    } else {
+      compute(o0);
     }
   }
-  for (int st = nfull * kChunk; st < nsteps; ++st) {
+  for (int st = nfull * kC; st < nsteps; ++st) {
     const int k0 = 512 * st;
-    bf8 a1[2], b1[2];
+    bf8 a1[4], b1[4];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) a1[i] = *reinterpret_cast<const bf8*>(ap[i] + k0);
+    for (int i = 0; i < 4; ++i) a1[i] = *reinterpret_cast<const bf8*>(ap[i] + k0);
 #pragma unroll
-    for (int j = 0; j < 2; ++j) b1[j] = *reinterpret_cast<const bf8*>(bp[j] + k0);
+    for (int j = 0; j < 4; ++j) b1[j] = *reinterpret_cast<const bf8*>(bp[j] + k0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) acc[i][j] = mfma32(a1[i], b1[j], acc[i][j]);
+      for (int j = 0; j < 4; ++j) acc[i][j] = mfma32(a1[i], b1[j], acc[i][j]);
   }
   float* out = J.part[l] + size_t(z) * N * K;
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < 4; ++j) {
       const int col = n0 + 16 * j + (lane & 15);
       if (col >= K) continue;
 #pragma unroll
@@ -872,12 +885,13 @@ static int mlp_backward(yh_mlp* m, const MlpArgs& a, const AdamStep* adam) {
   J.kc = kc;
   J.W = W;
   J.Mp = Mp;
+  J.tiles_m = (W + kWgTile - 1) / kWgTile;
   size_t poff = 0;
   J.first[0] = 0;
   for (int l = 0; l < L; ++l) {
     J.k[l] = m->fan_in(l);
-    J.tiles_k[l] = (J.k[l] + 63) / 64;
-    J.first[l + 1] = J.first[l] + J.tiles_k[l] * (W / 64) * nz;
+    J.tiles_k[l] = (J.k[l] + kWgTile - 1) / kWgTile;
+    J.first[l + 1] = J.first[l] + J.tiles_k[l] * J.tiles_m * nz;
     J.A[l] = hptr(*m->dzT[l]);
     J.B[l] = l == 0 ? hptr(m->xT) : hptr(*m->hT[l - 1]);
     J.part[l] = fptr(m->wsplit, poff);
