@@ -94,6 +94,35 @@ __device__ __forceinline__ void rot_group(typename Cplx<R>::V (&a)[16], int gb, 
   if (gb > 3) rot_local<R>(a, 3, u4 + 12);
 }
 
+// real rotation [[c, -s], [s, c]] on local bit k of 16 amplitudes (cs = (c, s))
+template <class R>
+__device__ __forceinline__ void rot_local_real(typename Cplx<R>::V (&a)[16], int k, typename Cplx<R>::V cs) {
+  using V = typename Cplx<R>::V;
+  const R c = cs.x, sn = cs.y;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    if (((i >> k) & 1) == 0) {
+      const int j = i | (1 << k);
+      const V w = a[i], v = a[j];
+      V o0, o1;
+      o0.x = fma(c, w.x, -sn * v.x);
+      o0.y = fma(c, w.y, -sn * v.y);
+      o1.x = fma(sn, w.x, c * v.x);
+      o1.y = fma(sn, w.y, c * v.y);
+      a[i] = o0;
+      a[j] = o1;
+    }
+  }
+}
+
+template <class V>
+__device__ __forceinline__ V cmul(V a, V b) {
+  V r;
+  r.x = fma(a.x, b.x, -(a.y * b.y));
+  r.y = fma(a.x, b.y, a.y * b.x);
+  return r;
+}
+
 __device__ __forceinline__ double bred(double v, double* sh) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -284,13 +313,16 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
                                                      const VqeBig* __restrict__ big,
                                                      const VqeStep* __restrict__ ent,
                                                      const VqeSeg* __restrict__ seg, int nseg, int64_t nitems,
-                                                     const typename Cplx<R>::V* __restrict__ rot,
+                                                     const typename Cplx<R>::V* __restrict__ ryph,
                                                      typename Cplx<R>::V* __restrict__ states,
                                                      double* __restrict__ partials) {
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
   constexpr int kPer = TN / NT;
   __shared__ V lds[TN + TN / 16];
+  __shared__ V phlo[1 << kVqeLow];         // phase of LDS position bits 0..3
+  __shared__ V phhi[1 << (TB - kVqeLow)];  // phase of position bits 4.., times the item's fixed bits
+  __shared__ V phq[64];                    // e^{i theta_q} of the layer being completed, per memory bit
   __shared__ double red[16];
   const int tid = threadIdx.x;
   int64_t w = blockIdx.x;
@@ -327,10 +359,41 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
         pre[k] = nxt.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[nxt.state_off + int64_t(x)];
       }
     }
-    // layer `layer` on tile positions first..tbits-1, in groups of 4 bits held in
-    // registers; with `cz` the CZ chain's sign is applied as the last group is stored
+    // layer `layer`'s Ry rotations on tile positions first..tbits-1, in groups of 4
+    // bits held in registers; with `cz` the layer is complete after them, and the
+    // diagonal that ends it (the CZ chain's sign times the merged Rz phases,
+    // prod over set bits q of e^{i theta_q}) is applied as the last group is stored,
+    // from two phase tables of this item
     auto rotate = [&](int layer, int first, bool cz) {
-      const V* ul = rot + cur.rot_off + int64_t(layer) * cur.n * 4;
+      const V* ul = ryph + cur.rot_off / 2 + int64_t(layer) * cur.n * 2;
+      if (cz) {
+        // memory bit of LDS position bit pb (see vqe_mem_index)
+        auto mbit = [&](int pb) {
+          if (pb < kVqeLow) return pb;
+          if (pb < cur.tbits) return cur.lo_bit + pb - kVqeLow;
+          const int j = pb - cur.tbits + kVqeLow;
+          return j < cur.lo_bit ? j : j + cur.nh;
+        };
+        // the layer's phases in LDS (one load each), then the two tables from them
+        for (int q = tid; q < cur.n; q += NT) phq[q] = ul[2 * q + 1];
+        lds_barrier();
+        if (tid < (1 << kVqeLow)) {
+          V f = cmk<V>(R(1), R(0));
+#pragma unroll
+          for (int b = 0; b < kVqeLow; ++b)
+            if ((tid >> b) & 1) f = cmul(f, phq[b]);
+          phlo[tid] = f;
+        }
+        const uint64_t x0 = vqe_mem_index<TB>(cur, 0);  // the item's fixed bits
+        for (int t = tid; t < (1 << (TB - kVqeLow)); t += NT) {
+          V f = cmk<V>(R(1), R(0));
+          for (uint64_t r = x0; r; r &= r - 1) f = cmul(f, phq[__builtin_ctzll(r)]);
+          for (int b = 0; b < TB - kVqeLow; ++b)
+            if ((t >> b) & 1) f = cmul(f, phq[mbit(b + kVqeLow)]);
+          phhi[t] = f;
+        }
+        lds_barrier();
+      }
       for (int g0 = first; g0 < cur.tbits; g0 += 4) {
         const int gb = min(4, cur.tbits - g0);
         const bool signs = cz && g0 + 4 >= cur.tbits;
@@ -350,15 +413,18 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
             if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
 #pragma unroll
           for (int k = 0; k < 4; ++k)
-            if (k < gb) rot_local<R>(a, k, ul + mbk[k] * 4);
+            if (k < gb) rot_local_real<R>(a, k, ul[mbk[k] * 2]);
 #pragma unroll
           for (int i = 0; i < 16; ++i)
             if (i < (1 << gb)) {
               const int p = base + (i << g0);
               V v = a[i];
-              if (signs && cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
-                v.x = -v.x;
-                v.y = -v.y;
+              if (signs) {
+                v = cmul(v, cmul(phlo[p & ((1 << kVqeLow) - 1)], phhi[p >> kVqeLow]));
+                if (cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
+                  v.x = -v.x;
+                  v.y = -v.y;
+                }
               }
               lds[vpad<TB>(p)] = v;
             }
@@ -416,6 +482,118 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
   }
 }
 
+// One instance of the compose launch: its records (one per (layer, bit), one
+// zero record at depth 0) start at rec0, its angles at ang.
+struct VqeRec {
+  int64_t rec0, ang;
+  int32_t n, depth;
+};
+
+// x conj(y)
+__device__ __forceinline__ double2 cmul_conj(double2 x, double2 y) {
+  return make_double2(x.x * y.x + x.y * y.y, x.y * y.x - x.x * y.y);
+}
+
+// The per-qubit layer matrix U = G_last ... G_first from its angles, in FP64, with
+// the gate matrices of the plain path: Ry(t) = [[cos t/2, -sin t/2], [sin t/2,
+// cos t/2]], Rz(t) = diag(e^{-i t/2}, e^{i t/2}); kinds Ry, Rz (apq = 2) or Rz, Ry, Rz
+// (apq = 3, general_u2).
+__device__ void vqe_layer_matrix(const double* a, int apq, double2 (&u)[4]) {
+  u[0] = make_double2(1, 0);
+  u[1] = make_double2(0, 0);
+  u[2] = make_double2(0, 0);
+  u[3] = make_double2(1, 0);
+  for (int gidx = 0; gidx < apq; ++gidx) {
+    const bool ry = apq == 2 ? gidx == 0 : gidx == 1;
+    double sn, cs;
+    sincos(a[gidx] / 2.0, &sn, &cs);
+    double2 nx[4];
+    if (ry) {  // [[c, -s], [s, c]] u
+      for (int c = 0; c < 2; ++c) {
+        nx[c] = make_double2(cs * u[c].x - sn * u[2 + c].x, cs * u[c].y - sn * u[2 + c].y);
+        nx[2 + c] = make_double2(sn * u[c].x + cs * u[2 + c].x, sn * u[c].y + cs * u[2 + c].y);
+      }
+    } else {  // diag(e^{-i t/2}, e^{i t/2}) u
+      const double2 e0 = make_double2(cs, -sn), e1 = make_double2(cs, sn);
+      for (int c = 0; c < 2; ++c) {
+        nx[c] = make_double2(e0.x * u[c].x - e0.y * u[c].y, e0.x * u[c].y + e0.y * u[c].x);
+        nx[2 + c] = make_double2(e1.x * u[2 + c].x - e1.y * u[2 + c].y, e1.x * u[2 + c].y + e1.y * u[2 + c].x);
+      }
+    }
+    for (int e = 0; e < 4; ++e) u[e] = nx[e];
+  }
+}
+
+// U = e^{i phi} D(alpha) Ry(beta) D(gamma), D(t) = diag(1, e^{i t}), Ry(beta) =
+// [[c, -s], [s, c]]: c = |U00|, s = |U10|, e^{i phi} = U00 / c, e^{i alpha} =
+// U10 e^{-i phi} / s, e^{i gamma} = -U01 e^{-i phi} / s. For a unitary U this
+// reproduces every entry (U11 = det U conj(U00)); where U00 is zero, phi = 0; where
+// U10 is zero only alpha + gamma is defined: gamma = 0, e^{i alpha} = U11 e^{-i phi} / c.
+// Each entry of a product of Ry and diagonal gates is one product term, so it has
+// the relative precision of FP64 however small it is.
+__device__ void vqe_decompose(const double2 (&u)[4], double& c, double& s, double2& ea, double2& eg) {
+  c = hypot(u[0].x, u[0].y);
+  s = hypot(u[2].x, u[2].y);
+  const double2 ph = c > 0 ? make_double2(u[0].x / c, u[0].y / c) : make_double2(1, 0);
+  if (s > 0) {
+    const double2 a = cmul_conj(u[2], ph), g = cmul_conj(make_double2(-u[1].x, -u[1].y), ph);
+    ea = make_double2(a.x / s, a.y / s);
+    eg = make_double2(g.x / s, g.y / s);
+  } else {
+    const double2 a = cmul_conj(u[3], ph);
+    ea = make_double2(a.x / c, a.y / c);
+    eg = make_double2(1, 0);
+  }
+}
+
+// Every (instance, layer, bit) record in one launch: the layer matrix for the
+// LDS-resident kernel (rot, 4 entries) and, for the step kernel (ryph, 2 entries),
+// (c, s) and e^{i theta}, theta = alpha_L + gamma_{L+1} (alpha_L for the last layer):
+// D(alpha) of layer L and D(gamma) of layer L + 1 are diagonal, commute with the CZ
+// chain between them, and merge into that one phase on bit value 1. D(gamma) of
+// layer 0 acts on |0...0> and the phases e^{i phi} are global, so both drop out.
+template <class R>
+__global__ __launch_bounds__(256) void k_vqe_compose(const VqeRec* __restrict__ recs, int ninst, int64_t nrec,
+                                                     const double* __restrict__ angles, int apq,
+                                                     typename Cplx<R>::V* __restrict__ rot,
+                                                     typename Cplx<R>::V* __restrict__ ryph) {
+  using V = typename Cplx<R>::V;
+  for (int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; r < nrec; r += int64_t(gridDim.x) * blockDim.x) {
+    int lo = 0, hi = ninst - 1;  // the instance whose records hold r
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (recs[mid].rec0 <= r) lo = mid;
+      else hi = mid - 1;
+    }
+    const VqeRec in = recs[lo];
+    if (in.depth == 0) {
+      for (int e = 0; e < 4; ++e) rot[4 * r + e] = cmk<V>(R(0), R(0));
+      ryph[2 * r] = cmk<V>(R(0), R(0));
+      ryph[2 * r + 1] = cmk<V>(R(0), R(0));
+      continue;
+    }
+    const int64_t k = r - in.rec0;
+    const int l = int(k / in.n), bit = int(k - int64_t(l) * in.n), loc = in.n - 1 - bit;
+    double2 u[4];
+    vqe_layer_matrix(angles + in.ang + (int64_t(l) * in.n + loc) * apq, apq, u);
+    for (int e = 0; e < 4; ++e) rot[4 * r + e] = cmk<V>(R(u[e].x), R(u[e].y));
+    double c, s;
+    double2 ea, eg;
+    vqe_decompose(u, c, s, ea, eg);
+    double2 th = ea;
+    if (l + 1 < in.depth) {
+      double2 un[4];
+      vqe_layer_matrix(angles + in.ang + (int64_t(l + 1) * in.n + loc) * apq, apq, un);
+      double c1, s1;
+      double2 ea1, eg1;
+      vqe_decompose(un, c1, s1, ea1, eg1);
+      th = make_double2(ea.x * eg1.x - ea.y * eg1.y, ea.x * eg1.y + ea.y * eg1.x);
+    }
+    ryph[2 * r] = cmk<V>(R(c), R(s));
+    ryph[2 * r + 1] = cmk<V>(R(th.x), R(th.y));
+  }
+}
+
 // energies of the HBM-resident instances: one workgroup per instance folds its
 // npass * 2^(n - TB) partial pairs in a fixed order
 template <int TB>
@@ -449,25 +627,6 @@ static const int* block_kinds(int apq) {
   return apq == 3 ? u2 : ryrz;
 }
 
-// Composed per-qubit layer matrix U = G_last * ... * G_first, FP64.
-static void compose_block(const double* a, int apq, cd u[4]) {
-  const int* kinds = block_kinds(apq);
-  cd acc[4] = {cd{1, 0}, cd{0, 0}, cd{0, 0}, cd{1, 0}};
-  for (int g = 0; g < apq; ++g) {
-    Mat m;
-    yh_op o{};
-    o.elem = YH_ELEM_GATE;
-    o.kind = kinds[g];
-    o.params[0] = a[g];
-    gate_matrix(o, m);
-    cd nxt[4];
-    for (int r = 0; r < 2; ++r)
-      for (int c = 0; c < 2; ++c) nxt[r * 2 + c] = m[r * 2 + 0] * acc[0 * 2 + c] + m[r * 2 + 1] * acc[1 * 2 + c];
-    for (int e = 0; e < 4; ++e) acc[e] = nxt[e];
-  }
-  for (int e = 0; e < 4; ++e) u[e] = acc[e];
-}
-
 struct VqeConfig {
   int tile_bits = 12;
   int blocks_per_cu = 2;
@@ -485,7 +644,9 @@ struct yh_vqe_batch {
   int max_instances = 0, max_qubits = 0, dtype = YH_C64;
   int cus = 256;
   Workspace w;
-  DevBuf insts, rot, energies, counter, big, steps, segs, states, partials;
+  DevBuf insts, rot, ryph, recs, angs, energies, counter, big, steps, segs, states, partials;
+  int64_t nrec = 0;  // compose records of the current batch
+  int apq = 2;
   VqeConfig cfg;
   // autotuned configuration per instance-shape hash (the batch's n and depth list)
   std::unordered_map<uint64_t, VqeConfig> tuned;
@@ -576,7 +737,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
 
 template <class R, int TB, int NT>
 static int vqe_steps_launch(yh_vqe_batch* b, const VqeConfig& cfg, const VqeInst* ins, const VqeBig* big,
-                            const VqeStep* ent, const VqeSeg* sg, const typename Cplx<R>::V* rot) {
+                            const VqeStep* ent, const VqeSeg* sg, const typename Cplx<R>::V* ryph) {
   using V = typename Cplx<R>::V;
   static int per_cu = 0;  // one per (R, TB, NT) instantiation
   if (per_cu == 0) {
@@ -586,7 +747,7 @@ static int vqe_steps_launch(yh_vqe_batch* b, const VqeConfig& cfg, const VqeInst
   const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
   for (const auto& l : b->launches)
     hipLaunchKernelGGL((k_vqe_step<R, TB, NT>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(NT), 0,
-                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, rot,
+                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, ryph,
                        static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
   return YH_OK;
 }
@@ -597,18 +758,22 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   int* ctr = static_cast<int*>(b->counter.p);
   const VqeInst* ins = static_cast<const VqeInst*>(b->insts.p);
   const V* rot = static_cast<const V*>(b->rot.p);
+  const V* ryph = static_cast<const V*>(b->ryph.p);
   double* en = static_cast<double*>(b->energies.p);
   const VqeBig* big = static_cast<const VqeBig*>(b->big.p);
   const VqeStep* ent = static_cast<const VqeStep*>(b->steps.p);
   const VqeSeg* sg = static_cast<const VqeSeg*>(b->segs.p);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, b->w.stream, ctr);
+  hipLaunchKernelGGL(k_vqe_compose<R>, dim3(unsigned(std::min<int64_t>((b->nrec + 255) / 256, 4096))), dim3(256), 0,
+                     b->w.stream, static_cast<const VqeRec*>(b->recs.p), ninst, b->nrec,
+                     static_cast<const double*>(b->angs.p), b->apq, const_cast<V*>(rot), const_cast<V*>(ryph));
   if (b->nbig < ninst)
     hipLaunchKernelGGL((k_vqe<R, TB>), dim3(b->cus * cfg.blocks_per_cu), dim3(256), 0, b->w.stream, ins, b->nbig,
                        ninst, ctr + 1, rot, en, J, h);
   if (cfg.step_threads == 512)
-    YH_TRY((vqe_steps_launch<R, TB, 512>(b, cfg, ins, big, ent, sg, rot)));
+    YH_TRY((vqe_steps_launch<R, TB, 512>(b, cfg, ins, big, ent, sg, ryph)));
   else
-    YH_TRY((vqe_steps_launch<R, TB, 256>(b, cfg, ins, big, ent, sg, rot)));
+    YH_TRY((vqe_steps_launch<R, TB, 256>(b, cfg, ins, big, ent, sg, ryph)));
   if (b->nbig > 0)
     hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
                        static_cast<const double*>(b->partials.p), en, J, h);
@@ -697,27 +862,25 @@ int yh_vqe_batch_destroy(yh_vqe_batch* b) {
   return YH_OK;
 }
 
-// Uploads the instance table and the FP64-composed layer matrices.
+// Uploads the instance table, one compose record per instance and the angles; the
+// layer matrices (and their Ry / phase form) are built on the device by
+// k_vqe_compose, the first launch of every batch.
 static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32_t* depth, const int64_t* offset,
                       const double* angles, int apq) {
   std::vector<VqeInst> ins(ninst);
-  std::vector<cd> rot;
+  std::vector<VqeRec> recs(ninst);
+  int64_t nrec = 0, nang = 0;
   for (int i = 0; i < ninst; ++i) {
     const int n = nq[i], d = depth[i];
     if (n < 1 || n > b->max_qubits || d < 0) return fail(YH_ERR_INVALID_ARG, "instance qubit count or depth out of range");
+    if (offset[i] < 0) return fail(YH_ERR_INVALID_ARG, "negative angle offset");
     ins[i].n = n;
     ins[i].depth = d;
     ins[i].out = i;
-    ins[i].rot_off = int64_t(rot.size());
-    for (int l = 0; l < d; ++l)
-      for (int bit = 0; bit < n; ++bit) {
-        const int loc = n - 1 - bit;
-        const double* a = angles + offset[i] + (int64_t(l) * n + loc) * apq;
-        cd u[4];
-        compose_block(a, apq, u);
-        rot.insert(rot.end(), u, u + 4);
-      }
-    if (d == 0) rot.insert(rot.end(), 4, cd{0, 0});  // keep rot_off valid
+    ins[i].rot_off = 4 * nrec;  // 4 rot entries, 2 ryph entries per record
+    recs[i] = VqeRec{nrec, offset[i], n, d};
+    nrec += d > 0 ? int64_t(d) * n : 1;  // one zero record keeps rot_off valid at depth 0
+    nang = std::max<int64_t>(nang, offset[i] + int64_t(apq) * n * d);
   }
   // qubit count descending (so any tile size splits the table into an HBM-resident
   // prefix and an LDS-resident suffix), then cost descending (longest-processing-time order)
@@ -737,16 +900,13 @@ static int vqe_upload(yh_vqe_batch* b, int ninst, const int32_t* nq, const int32
     b->sorted_d[k] = sorted[k].depth;
   }
   YH_TRY(b->w.upload(sorted.data(), sorted.size() * sizeof(VqeInst), b->insts));
-  if (b->dtype == YH_C128) {
-    YH_TRY(b->w.upload(rot.data(), rot.size() * sizeof(cd), b->rot));
-  } else {
-    std::vector<float> f(2 * rot.size());
-    for (size_t k = 0; k < rot.size(); ++k) {
-      f[2 * k] = float(rot[k].re);
-      f[2 * k + 1] = float(rot[k].im);
-    }
-    YH_TRY(b->w.upload(f.data(), f.size() * sizeof(float), b->rot));
-  }
+  YH_TRY(b->w.upload(recs.data(), recs.size() * sizeof(VqeRec), b->recs));
+  YH_TRY(b->w.upload(angles, size_t(std::max<int64_t>(nang, 1)) * sizeof(double), b->angs));
+  const size_t amp = b->dtype == YH_C128 ? 16 : 8;
+  YH_TRY(b->rot.ensure(size_t(nrec) * 4 * amp));
+  YH_TRY(b->ryph.ensure(size_t(nrec) * 2 * amp));
+  b->nrec = nrec;
+  b->apq = apq;
   return YH_OK;
 }
 
@@ -796,8 +956,9 @@ int yh_vqe_batch_energies(yh_vqe_batch* b, int ninst, const int32_t* nq, const i
   // replay a captured graph of the launch sequence; the key covers the instance
   // shapes, the configuration and every device buffer the launches read or write
   uint64_t key = b->plan_key ^ (uint64_t(b->cfg.blocks_per_cu) << 56) ^ (uint64_t(b->cfg.resident_grid) << 62) ^
-                 (uint64_t(b->cfg.step_threads) << 40);
-  for (const void* p : {b->insts.p, b->rot.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->segs.p,
+                 (uint64_t(b->cfg.step_threads) << 40) ^ (uint64_t(b->apq) << 32);
+  key = (key ^ uint64_t(b->nrec)) * 1099511628211ull;
+  for (const void* p : {b->insts.p, b->rot.p, b->ryph.p, b->recs.p, b->angs.p, b->energies.p, b->counter.p, b->big.p, b->steps.p, b->segs.p,
                         b->states.p, b->partials.p})
     key = (key ^ uint64_t(reinterpret_cast<uintptr_t>(p))) * 1099511628211ull;
   if (!b->graph || b->graph_key != key) {
