@@ -563,6 +563,7 @@ PROFILE_WORKLOADS = {
     "dm": "noisy brickwork on a 2^nbits x 2^nbits density matrix, per-gate fused passes",
     "dmblk": "the same on the superoperator block path",
     "mlp_train": "surrogate training: 2 epochs over 65536 random pairs, 288 angles, batch 4096",
+    "mlp_grad": "surrogate loss and gradient (no update, no graph) of one 4096-row batch, 288 angles, 40 times",
 }
 
 
@@ -621,6 +622,16 @@ def profile_workload(w, n, depth):
         t0 = time.perf_counter()
         losses = m.train(x, y, batch=4096, epochs=2, lr=1e-3, seed=3)
         print(f"[prof] mlp_train 2 epochs {time.perf_counter() - t0:.3f} s losses {losses}", flush=True)
+    elif w == "mlp_grad":
+        from yaohip.surrogate import Mlp
+        rng = np.random.default_rng(5)
+        x = rng.uniform(-0.8, 0.8, size=(4096, 288)).astype(np.float32)
+        y = np.cos(x[:, :8]).sum(axis=1).astype(np.float32)
+        m = Mlp(288, width=256, hidden=4, max_batch=4096, seed=1, encoding="trig")
+        t0 = time.perf_counter()
+        for _ in range(40):
+            loss, _g = m.loss_grad(x, y)
+        print(f"[prof] mlp_grad 40 x {time.perf_counter() - t0:.3f} s loss {loss}", flush=True)
 
 
 if __name__ == "__main__":
