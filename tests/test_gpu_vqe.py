@@ -1,7 +1,9 @@
 """Batched VQE energies (one persistent launch for many instances of uneven size)
 against the FP64 oracle and against the plain per-circuit GPU path.
-Bars: complex128 batch within 1e-10 of the oracle; complex64 batch |dE| reported,
-bounded by 1e-4 here (the bench states which precision meets |dE| <= 1e-6)."""
+Bars: complex128 batch within 1e-10 of the oracle; complex64 batch within 1e-5 (it
+stores and rotates amplitudes in FP32: about depth x n roundings of 6e-8 per
+amplitude, so |dE| grows like 6e-8 sqrt(depth n) |E|; 3.5e-6 was the largest of 4096
+instances of 10..20 qubits, and the bench reports the fraction within 1e-6)."""
 import numpy as np
 import pytest
 
@@ -38,7 +40,7 @@ def test_vqe_batch_c64_error_distribution():
     got = VqeBatch(64, 20, "c64").energies(inst)
     d = np.abs(got - want)
     print(f"c64 batch |dE|: max {d.max():.3e} median {np.median(d):.3e} p95 {np.percentile(d, 95):.3e}")
-    assert d.max() < 1e-4
+    assert d.max() < 1e-5
 
 
 def test_vqe_batch_vs_plain_path_large_instances():
@@ -97,7 +99,29 @@ def test_vqe_batch_rz_ry_rz_large_vs_plain():
     got = VqeBatch(64, 20, "c128").energies(inst)
     assert np.max(np.abs(got - plain)) < 1e-10
     got64 = VqeBatch(64, 20, "c64").energies(inst)
-    assert np.max(np.abs(got64 - plain)) < 1e-4
+    assert np.max(np.abs(got64 - plain)) < 1e-5
+
+
+@pytest.mark.parametrize("apq", [2, 3])
+def test_vqe_batch_exact_zero_entries_vs_plain(apq):
+    """Angles of exactly 0 and +-pi make entries of a layer matrix zero (Ry(0) has no
+    off-diagonal part; a Ry(pi) has a cosine of 6e-17): the step kernel's Ry / phase
+    form of each layer matrix must still reproduce it, on instances that stream
+    through HBM tiles (16 and 17 qubits) and on LDS-resident ones (9 qubits)."""
+    from yaohip.vqe import fixed_instances
+    rng = np.random.default_rng(21)
+    for n, depth in ((16, 4), (17, 3), (9, 5)):
+        a = rng.uniform(-np.pi, np.pi, size=(6, depth * n * apq))
+        a[0] = 0.0
+        a[1, ::2] = 0.0
+        a[2, ::3] = np.pi
+        a[3, 1::3] = -np.pi
+        a[4] = np.where(rng.random(a.shape[1]) < 0.5, 0.0, np.pi)
+        inst = fixed_instances(a, n, depth, apq)
+        plain = plain_energies(inst, dtype="c128")
+        got = VqeBatch(16, 20, "c128").energies(inst)
+        print(f"n={n} depth={depth} apq={apq}: max |dE| vs plain {np.max(np.abs(got - plain)):.3e}")
+        assert np.max(np.abs(got - plain)) < 1e-10
 
 
 def test_vqe_batch_rejects_bad_angle_count():
