@@ -269,6 +269,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   struct Geo {
     int q[kBQ];
     int cb, cb1, in_lane, out_lane;
+    int xb0, xb1, ob0, ob1;  // padded LDS index of the lane's column | row bits 0..1, operands / results
     bool small;
   };
   auto roff = [](const Geo& g, int r) {
@@ -279,10 +280,12 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   };
   auto geo = [&](int b, Geo& g) {
     int qs[kBQ];
+    // the block's bit positions are the same in every lane: scalar registers, so
+    // the uniform row offsets below are scalar arithmetic
 #pragma unroll
     for (int j = 0; j < kBQ; ++j) {
-      g.q[j] = bq[b][j];
-      qs[j] = bq[b][kBQ + j];
+      g.q[j] = __builtin_amdgcn_readfirstlane(bq[b][j]);
+      qs[j] = __builtin_amdgcn_readfirstlane(bq[b][kBQ + j]);
     }
     g.small = NTL == 2 || ((a.small >> b) & 1);
     g.in_lane = roff(g, grp);
@@ -300,6 +303,10 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
       g.cb = at(wave * kCols + col);
       g.cb1 = 0;
     }
+    g.xb0 = tpad(g.cb | g.in_lane);
+    g.xb1 = tpad(g.cb1 | g.in_lane);
+    g.ob0 = tpad(g.cb | g.out_lane);
+    g.ob1 = tpad(g.cb1 | g.out_lane);
   };
   // tile position of operand m (B fragment: k-step m, or group m / 4 and k-step
   // m % 4) and of result m (row tile m / 4, or group m / 4; element m % 4)
@@ -309,6 +316,16 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
   auto opos = [&](const Geo& g, int m) {
     const int d = M::drow(lane, m & 3) - M::drow(lane, 0);
     return g.small ? ((m >> 2 ? g.cb1 : g.cb) | g.out_lane | roff(g, d)) : (g.cb | g.out_lane | roff(g, 16 * (m >> 2) + d));
+  };
+  // their padded LDS indices: p + p / 16 is additive over disjoint bit fields, so
+  // each index is the lane's base (column and row bits 0..1) plus a row offset that
+  // is the same in every lane
+  auto xlds = [&](const Geo& g, int m) {
+    return g.small ? (m >> 2 ? g.xb1 : g.xb0) + tpad(roff(g, 4 * (m & 3))) : g.xb0 + tpad(roff(g, 4 * m));
+  };
+  auto olds = [&](const Geo& g, int m) {
+    const int d = M::drow(lane, m & 3) - M::drow(lane, 0);
+    return g.small ? (m >> 2 ? g.ob1 : g.ob0) + tpad(roff(g, d)) : g.ob0 + tpad(roff(g, 16 * (m >> 2) + d));
   };
   // a workgroup step takes NTL adjacent tiles (a "unit"); units are dealt out in
   // turn to the persistent workgroups
@@ -358,7 +375,7 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
         for (int m = 0; m < 8; ++m) x[m] = cmk<V>(vx[i][m], vy[i][m]);
       } else {
 #pragma unroll
-        for (int m = 0; m < 8; ++m) x[m] = lds[i][tpad(xpos(g, m))];
+        for (int m = 0; m < 8; ++m) x[m] = lds[i][xlds(g, m)];
       }
     };
     // three real products instead of four (Gauss): P1 = Ur Sr, P2 = Ui Si,
@@ -379,9 +396,8 @@ __global__ __launch_bounds__(kTThreads, 1) void k_tblocks(typename Cplx<R>::V* _
         const R a1 = p1[m >> 2][m & 3], a2 = p2[m >> 2][m & 3];
         o.x = a1 - a2;
         o.y = p3[m >> 2][m & 3] - a1 - a2;
-        const int p = opos(g, m);
-        if (to_hbm) s[tb + mem(p)] = o;
-        else lds[i][tpad(p)] = o;
+        if (to_hbm) s[tb + mem(opos(g, m))] = o;
+        else lds[i][olds(g, m)] = o;
       }
     };
     auto zero = [&]() __attribute__((always_inline)) {
