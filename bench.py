@@ -564,6 +564,8 @@ PROFILE_WORKLOADS = {
     "dmblk": "the same on the superoperator block path",
     "mlp_train": "surrogate training: 2 epochs over 65536 random pairs, 288 angles, batch 4096",
     "mlp_grad": "surrogate loss and gradient (no update, no graph) of one 4096-row batch, 288 angles, 40 times",
+    "sample": "1,024 shots from every basis state's probability (sample_from_probs) of a deterministic 12-qubit "
+              "and 22-qubit state, 20 times each",
 }
 
 
@@ -622,6 +624,14 @@ def profile_workload(w, n, depth):
         t0 = time.perf_counter()
         losses = m.train(x, y, batch=4096, epochs=2, lr=1e-3, seed=3)
         print(f"[prof] mlp_train 2 epochs {time.perf_counter() - t0:.3f} s losses {losses}", flush=True)
+    elif w == "sample":
+        for nb in (12, 22):
+            reg = yaohip.ArrayReg.deterministic_state(nb)
+            u = np.random.default_rng(nb).random(1024)
+            t0 = time.perf_counter()
+            for _ in range(20):
+                out = yaohip.sample(reg, None, u)
+            print(f"[prof] sample n={nb} 20 x 1024 shots {time.perf_counter() - t0:.4f} s first {int(out[0])}", flush=True)
     elif w == "mlp_grad":
         from yaohip.surrogate import Mlp
         rng = np.random.default_rng(5)
