@@ -104,11 +104,38 @@ def test_brickwork28_complex64_errors_per_result(gpu):
     s64 = r64.state_vec()
     del r64
     res["amp_max_rel"] = max_rel(s64, s128)
-    res["fidelity_loss"] = 1.0 - abs(np.vdot(s128, s64)) ** 2
+    n64, n128 = np.vdot(s64, s64).real, np.vdot(s128, s128).real
+    res["norm_dev"] = abs(n64 - 1.0)
+    res["fidelity_loss"] = 1.0 - abs(np.vdot(s128, s64)) ** 2 / (n64 * n128)
     print(f"[case] complex64 errors: {res}", flush=True)
     assert res["amp_max_rel"] < 1e-5
-    assert res["fidelity_loss"] < 1e-9
+    assert 0 <= res["fidelity_loss"] < 1e-9 and res["norm_dev"] < 1e-6
     assert all(res[k] < 1e-6 for k in res if k.startswith("zz") or k.startswith("marg"))
+
+
+@pytest.mark.parametrize("fused", [0, 4, 5])
+def test_qft12_complex64_errors_per_result(gpu, fused):
+    """The 12-qubit QFT with SWAPs on a seeded state in complex64 (gate by gate, one LDS
+    tile pass, FP32 MFMA blocks) against the oracle (FP64): every reported result's
+    error, per result."""
+    from conftest import seeded_state
+    from yaohip.workloads import qft_with_swaps
+    n = 12
+    psi = seeded_state(n, 2024)
+    els = qft_with_swaps(n)
+    want = O.apply(n, psi, lower(els))
+    reg = gpu.ArrayReg.from_vec(n, psi, "c64")
+    gpu.apply_inplace(Circuit.qubits(n, els), reg, fused)
+    got = reg.state_vec()
+    res = {"amp_max_rel": max_rel(got, want),
+           "fidelity_loss": 1.0 - abs(np.vdot(want, got)) ** 2 / (np.vdot(want, want).real * np.vdot(got, got).real)}
+    for i in (0, 5, 10):
+        res[f"zz{i}"] = abs(gpu.expect_arrayreg(reg, zz(i)) - O.expect(n, want, zz(i)).real)
+    for locs in ([0, 1], list(range(12)), [11, 3, 7]):
+        res[f"marg{locs if len(locs) < 12 else 'all'}"] = float(np.max(np.abs(gpu.probs(reg, locs) - O.probs(n, want, locs))))
+    print(f"[case] qft12 complex64 fused={fused} errors: {res}", flush=True)
+    assert res["amp_max_rel"] < 1e-6
+    assert all(v < 1e-6 for k, v in res.items() if k != "amp_max_rel")
 
 
 def test_noisy_dm14_depth10_blocks_vs_per_gate(gpu):
