@@ -231,6 +231,89 @@ def test_sample_bit_exact_cdf_chunks(gpu, n):
         assert np.array_equal(got, O.sample(n, psi, None, u))
 
 
+def _adversarial_distributions():
+    """States whose probabilities |a|^2 are exact dyadic numbers, chosen against the
+    parallel form of the sequential cumulative sum: basis states (one bucket of 1,
+    the rest zero), GHZ, uniform, runs of exact half-ulp ties before and after the sum
+    changes binade (also at a chunk edge of 4096), subnormal probabilities and
+    probabilities near 2^600; up to 2^14 buckets (one workgroup) and above."""
+    out = []
+    n = 12
+    N = 1 << n
+    for k in (0, 5, N - 1):
+        z = np.zeros(N, complex)
+        z[k] = 1
+        out.append((f"basis {k}", n, z))
+    z = np.zeros(N, complex)
+    z[0] = z[-1] = 0.5 + 0.5j  # p = 0.5 each
+    out.append(("ghz", n, z))
+    out.append(("uniform", n, np.full(N, 2.0 ** -6 + 0j)))
+    z = np.full(N, 2.0 ** -27 + 2.0 ** -27 * 1j)  # p = 2^-53: half an ulp of 1.0
+    z[0] = 1
+    out.append(("ties after 1", n, z))
+    # p in {0, 2^-54, 2^-53, 2^-52, 2^-50 + 2^-54}: ties at 2^-54 while the sum is in
+    # [0.5, 1), at 2^-53 in [1, 2) and at 2^-52 in [2, 4); the 0.5 and 1 buckets move it
+    n = 13
+    N = 1 << n
+    vals = np.array([0, 2.0 ** -27, 2.0 ** -27 + 2.0 ** -27 * 1j, 2.0 ** -26, 2.0 ** -25 + 2.0 ** -27 * 1j])
+    rng = np.random.default_rng(77)
+    z = vals[rng.integers(0, len(vals), N)].astype(complex)
+    z[0] = 0.5 + 0.5j
+    z[4096] = 0.5 + 0.5j
+    z[6000] = 1
+    out.append(("mixed ties", n, z))
+    z = np.zeros(N, complex)
+    z[rng.integers(0, N, 300)] = 2.0 ** -530  # p = 2^-1060, subnormal
+    z[rng.integers(0, N, 30)] = 2.0 ** -511
+    z[-5] = 2.0 ** -20
+    out.append(("subnormal", n, z))
+    out.append(("scaled 2^600", 12, seeded_state(12, 5) * 2.0 ** 300))
+    z = seeded_state(14, 6)
+    z[rng.random(1 << 14) < 0.3] = 0  # zero runs across chunks
+    out.append(("random with zeros", 14, z))
+    # above 16384 buckets the chunks are planned together and walked by one workgroup
+    n = 16
+    N = 1 << n
+    z = vals[rng.integers(0, len(vals), N)].astype(complex)
+    z[0] = 0.5 + 0.5j
+    z[4096 * 5] = 0.5 + 0.5j
+    z[4096 * 9 + 17] = 1
+    out.append(("mixed ties, 16 chunks", n, z))
+    z = np.full(1 << 15, 2.0 ** -27 + 2.0 ** -27 * 1j)  # every chunk folded bucket by bucket
+    z[3] = 1
+    out.append(("ties after 1, 8 chunks", 15, z))
+    z = np.zeros(1 << 16, complex)
+    z[(1 << 16) - 1] = 1
+    out.append(("basis last, 16 chunks", 16, z))
+    z = seeded_state(17, 7)
+    z[rng.random(1 << 17) < 0.5] = 0
+    z[: 1 << 13] = 0
+    out.append(("random with zeros, 32 chunks", 17, z))
+    return out
+
+
+@pytest.mark.parametrize("case", range(14))
+def test_sample_bit_exact_adversarial(gpu, case):
+    """Every partial sum of the device cumulative sum equals the sequential fold's:
+    uniforms at each partial sum and just below it pick the same bucket as the
+    oracle's loop (measure.rs:140-150)."""
+    name, n, z = _adversarial_distributions()[case]
+    print(f"[case] cdf adversarial '{name}' n={n}", flush=True)
+    reg = gpu.ArrayReg.from_vec(n, z)
+    p = O.probs(n, z)
+    cs = np.empty(len(p))
+    acc = 0.0
+    for i, x in enumerate(p):
+        acc += x
+        cs[i] = acc
+    u = np.concatenate([cs, np.nextafter(cs, 0.0), np.random.default_rng(case).random(500) * cs[-1]])
+    got = gpu.sample(reg, None, u)
+    want = np.searchsorted(cs, u, side="right")
+    want[want >= len(p)] = len(p) - 1
+    assert np.array_equal(got, want)
+    assert np.array_equal(got, O.sample(n, z, None, u))
+
+
 @pytest.mark.parametrize("post", [0, 1, 2])
 def test_measure_postprocess_vs_oracle(gpu, post):
     n = 8

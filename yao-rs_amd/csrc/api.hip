@@ -1075,7 +1075,9 @@ static int sample_dev(yh_reg* reg, const int32_t* locs, int nlocs, const double*
   YH_TRY(w.probs.ensure(2 * m * sizeof(double)));
   double* p = static_cast<double*>(w.probs.p);
   YH_TRY(probs_to_dev(reg, locs, nlocs, p));
-  YH_TRY(launch_cdf(p, m, p + m, w.stream));
+  const size_t csb = cdf_scratch_bytes(m);
+  if (csb) YH_TRY(w.cdfws.ensure(csb));
+  YH_TRY(launch_cdf(p, m, p + m, csb ? w.cdfws.p : nullptr, w.stream));
   YH_TRY(w.upload(uniforms, nshots * sizeof(double), w.shots));
   YH_TRY(w.shots_out.ensure(nshots * sizeof(uint64_t)));
   YH_TRY(launch_search(p + m, m, static_cast<const double*>(w.shots.p), nshots,

@@ -96,8 +96,10 @@ int probs_marginal_dev(const void* s, int dtype, int nbits, const int32_t* locbi
 int dm_probs_dev(const double2* rho, int nbits, const int32_t* locbits_dev,
                  const int32_t* locbits, int nlocs, double* out, double* ws, uint64_t nchunks,
                  hipStream_t st);
-// sequential cumulative sum (single thread) and per-shot search
-int launch_cdf(const double* p, uint64_t n, double* cdf, hipStream_t st);
+// cumulative sum with the bits of the sequential left fold (scratch: cdf_scratch_bytes(n)
+// bytes of device memory, none up to 4096 buckets) and per-shot search
+size_t cdf_scratch_bytes(uint64_t n);
+int launch_cdf(const double* p, uint64_t n, double* cdf, void* scratch, hipStream_t st);
 int launch_search(const double* cdf, uint64_t n, const double* r, uint64_t nshots, uint64_t* out,
                   hipStream_t st);
 // collapse to outcome (zero the rest), reset permutation, removed-qubit gather

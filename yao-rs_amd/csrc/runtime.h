@@ -42,6 +42,7 @@ struct Workspace {
   DevBuf probs;      // probabilities and their CDF
   DevBuf shots;      // caller-supplied uniforms
   DevBuf shots_out;  // sampled outcomes
+  DevBuf cdfws;      // cumulative-sum records of large distributions
   DevBuf tiles;      // tile-pass groups and micro gates
   DevBuf acc;        // density matrix: sum of Kraus branches (copied into rho)
   DevBuf branch;     // density matrix: one Kraus branch
