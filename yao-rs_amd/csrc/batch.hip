@@ -256,7 +256,16 @@ struct VqeItem {
   int64_t item;  // index within the instance's step
   int inst, n, depth, step, last, chunk, npass, nh, lo_bit, tbits;
   int64_t rot_off, state_off, part_off;
+  uint32_t m0;  // memory index of tile position 0: the item's fixed bits
 };
+
+// the same value in every lane, in scalar registers
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+  const uint32_t lo = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(x))));
+  const uint32_t hi = uint32_t(__builtin_amdgcn_readfirstlane(int(uint32_t(uint64_t(x) >> 32))));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
 
 // chunk held by step j of k chunks (k >= 2): 0, 1, .., k-1, k-2, .., 1, 0, 1, ..
 __host__ __device__ inline int vqe_bounce(int j, int k) {
@@ -291,18 +300,41 @@ __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big
     it.nh = min(cs, it.n - it.lo_bit);
   }
   it.tbits = kVqeLow + it.nh;
+  // every field is the same in all lanes: scalar registers
+  it.item = uni64(it.item);
+  it.inst = uni(it.inst);
+  it.n = uni(it.n);
+  it.depth = uni(it.depth);
+  it.step = uni(it.step);
+  it.last = uni(it.last);
+  it.chunk = uni(it.chunk);
+  it.npass = uni(it.npass);
+  it.nh = uni(it.nh);
+  it.lo_bit = uni(it.lo_bit);
+  it.tbits = uni(it.tbits);
+  it.rot_off = uni64(it.rot_off);
+  it.state_off = uni64(it.state_off);
+  it.part_off = uni64(it.part_off);
+  // fixed bits: the item index above the tile's positions, with zeros inserted at 0..3
+  // and at the chunk
+  const uint64_t y = (uint64_t(it.item) << (TB - it.tbits)) << kVqeLow;
+  it.m0 = uint32_t(((y >> it.lo_bit) << (it.lo_bit + it.nh)) | (y & ((uint64_t(1) << it.lo_bit) - 1)));
 }
 
-// memory index of LDS position p: tile (item << (TB - tbits)) + (p >> tbits) fills the
-// bits outside the tile in ascending order (zeros inserted at 0..3 and at the chunk)
+// The memory index of tile position p is m0 | vqe_spread(p): positions 0..3 are memory
+// bits 0..3, positions 4..tbits-1 the chunk's bits lo_bit.., and positions from tbits
+// on (a chunk narrower than the tile's TB - 4 bits) memory bits 4, 5, .. below the
+// chunk. A bit-spreading map: vqe_spread(a | b) = vqe_spread(a) | vqe_spread(b) for
+// disjoint a, b, so a thread's positions tid + NT k take one per-lane part and one
+// uniform part. Instances have at most 24 qubits, so indices fit 32 bits.
 template <int TB>
-__device__ __forceinline__ uint64_t vqe_mem_index(const VqeItem& it, int p) {
-  const uint64_t t = (uint64_t(it.item) << (TB - it.tbits)) + uint64_t(p >> it.tbits);
-  uint64_t y = t << kVqeLow;
-  y = ((y >> it.lo_bit) << (it.lo_bit + it.nh)) | (y & ((uint64_t(1) << it.lo_bit) - 1));
-  const uint64_t q = uint64_t(p & ((1 << it.tbits) - 1));
-  return y | (q & ((1u << kVqeLow) - 1)) | ((q >> kVqeLow) << it.lo_bit);
+__device__ __forceinline__ uint32_t vqe_spread(const VqeItem& it, int p) {
+  const uint32_t l = uint32_t(p) & ((1u << kVqeLow) - 1);
+  const uint32_t q = (uint32_t(p) >> kVqeLow) & ((1u << it.nh) - 1);
+  const uint32_t h = uint32_t(p) >> it.tbits;
+  return l | (q << it.lo_bit) | (h << kVqeLow);
 }
+
 
 // Persistent: workgroup g takes items g, g + grid, ...; the next item's segment is
 // found by a forward scan over the launch's few segments, its entry by one shift,
@@ -319,10 +351,19 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
   using V = typename Cplx<R>::V;
   constexpr int TN = 1 << TB;
   constexpr int kPer = TN / NT;
+  constexpr int kHi = 1 << (TB - 3);
   __shared__ V lds[TN + TN / 16];
-  __shared__ V phlo[1 << kVqeLow];         // phase of LDS position bits 0..3
-  __shared__ V phhi[1 << (TB - kVqeLow)];  // phase of position bits 4.., times the item's fixed bits
-  __shared__ V phq[64];                    // e^{i theta_q} of the layer being completed, per memory bit
+  // The diagonal that ends a layer, at tile position p: the CZ chain's sign times the
+  // merged Rz phases of the set bits. The sign (-1)^(sum of x_i x_(i+1)) splits into
+  // the pairs among memory bits 0..3 (position bits 0..3), the pair (3, 4) and the
+  // pairs from bit 4 up, which depend on position bits 3.. and the item's fixed bits
+  // only; so the diagonal is dlo[p & 15] * dhi[p >> 3]. ZZ splits the same way
+  // (zlo, zhi: counts of unequal neighbouring bits).
+  __shared__ V dlo[1 << kVqeLow];
+  __shared__ V dhi[kHi];
+  __shared__ uint8_t zlo[1 << kVqeLow];
+  __shared__ uint8_t zhi[kHi];
+  __shared__ V phq[64];  // e^{i theta_q} of the layer being completed, per memory bit
   __shared__ double red[16];
   const int tid = threadIdx.x;
   int64_t w = blockIdx.x;
@@ -336,14 +377,18 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
   };
   VqeItem cur, nxt;
   locate(w, cur);
-  // memory indices are recomputed where needed (a few integer operations) rather
-  // than kept in 64-bit registers for the whole item
+  // a thread's positions tid + NT k: memory index m0 | spread(tid) | spread(NT k)
   V pre[kPer];
+  auto load_item = [&](const VqeItem& it) __attribute__((always_inline)) {
+    const uint32_t lane_bits = vqe_spread<TB>(it, tid);
+    const V* src = states + it.state_off;
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const uint64_t x = vqe_mem_index<TB>(cur, tid + NT * k);
-    pre[k] = cur.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[cur.state_off + int64_t(x)];
-  }
+    for (int k = 0; k < kPer; ++k) {
+      const uint32_t x = it.m0 | lane_bits | vqe_spread<TB>(it, NT * k);
+      pre[k] = it.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : src[x];
+    }
+  };
+  load_item(cur);
   for (;;) {
 #pragma unroll
     for (int k = 0; k < kPer; ++k) lds[vpad<TB>(tid + NT * k)] = pre[k];
@@ -353,28 +398,23 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
     const bool more = wn < nitems;
     if (more) {
       locate(wn, nxt);
-#pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const uint64_t x = vqe_mem_index<TB>(nxt, tid + NT * k);
-        pre[k] = nxt.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : states[nxt.state_off + int64_t(x)];
-      }
+      load_item(nxt);
     }
+    // pairs of neighbouring memory bits from bit 4 up, of the memory index of positions
+    // (u << 3) (u = p >> 3): mask of the pairs (i, i + 1) with 4 <= i <= n - 2, and the
+    // pair (3, 4) from position bit 3 (bit 0 of u) and memory bit 4
+    auto hi_bits = [&](int u, uint32_t& y, uint32_t& b3, uint32_t& x4) {
+      y = cur.m0 | vqe_spread<TB>(cur, (u >> 1) << kVqeLow);
+      b3 = uint32_t(u & 1);
+      x4 = (y >> kVqeLow) & 1;
+    };
+    const uint32_t pmask = ((1u << (cur.n - 1)) - 1) & ~((1u << kVqeLow) - 1);
     // layer `layer`'s Ry rotations on tile positions first..tbits-1, in groups of 4
     // bits held in registers; with `cz` the layer is complete after them, and the
-    // diagonal that ends it (the CZ chain's sign times the merged Rz phases,
-    // prod over set bits q of e^{i theta_q}) is applied as the last group is stored,
-    // from two phase tables of this item
+    // diagonal that ends it is applied as the last group is stored
     auto rotate = [&](int layer, int first, bool cz) {
       const V* ul = ryph + cur.rot_off / 2 + int64_t(layer) * cur.n * 2;
       if (cz) {
-        // memory bit of LDS position bit pb (see vqe_mem_index)
-        auto mbit = [&](int pb) {
-          if (pb < kVqeLow) return pb;
-          if (pb < cur.tbits) return cur.lo_bit + pb - kVqeLow;
-          const int j = pb - cur.tbits + kVqeLow;
-          return j < cur.lo_bit ? j : j + cur.nh;
-        };
-        // the layer's phases in LDS (one load each), then the two tables from them
         for (int q = tid; q < cur.n; q += NT) phq[q] = ul[2 * q + 1];
         lds_barrier();
         if (tid < (1 << kVqeLow)) {
@@ -382,15 +422,16 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
 #pragma unroll
           for (int b = 0; b < kVqeLow; ++b)
             if ((tid >> b) & 1) f = cmul(f, phq[b]);
-          phlo[tid] = f;
+          if (__popc(tid & (tid >> 1)) & 1) f = cmk<V>(-f.x, -f.y);
+          dlo[tid] = f;
         }
-        const uint64_t x0 = vqe_mem_index<TB>(cur, 0);  // the item's fixed bits
-        for (int t = tid; t < (1 << (TB - kVqeLow)); t += NT) {
+        for (int u = tid; u < kHi; u += NT) {
+          uint32_t y, b3, x4;
+          hi_bits(u, y, b3, x4);
           V f = cmk<V>(R(1), R(0));
-          for (uint64_t r = x0; r; r &= r - 1) f = cmul(f, phq[__builtin_ctzll(r)]);
-          for (int b = 0; b < TB - kVqeLow; ++b)
-            if ((t >> b) & 1) f = cmul(f, phq[mbit(b + kVqeLow)]);
-          phhi[t] = f;
+          for (uint32_t r = y >> kVqeLow; r; r &= r - 1) f = cmul(f, phq[__builtin_ctz(r) + kVqeLow]);
+          if ((__popc(y & (y >> 1) & pmask) + (b3 & x4)) & 1) f = cmk<V>(-f.x, -f.y);
+          dhi[u] = f;
         }
         lds_barrier();
       }
@@ -419,13 +460,7 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
             if (i < (1 << gb)) {
               const int p = base + (i << g0);
               V v = a[i];
-              if (signs) {
-                v = cmul(v, cmul(phlo[p & ((1 << kVqeLow) - 1)], phhi[p >> kVqeLow]));
-                if (cz_sign(vqe_mem_index<TB>(cur, p), cur.n) < 0) {
-                  v.x = -v.x;
-                  v.y = -v.y;
-                }
-              }
+              if (signs) v = cmul(v, cmul(dlo[p & ((1 << kVqeLow) - 1)], dhi[p >> 3]));
               lds[vpad<TB>(p)] = v;
             }
         }
@@ -450,16 +485,49 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
       // per (chunk, item)
       const bool first_e = j == cur.last;
       const int xfirst = first_e ? 0 : kVqeLow;
+      if (first_e) {
+        if (tid < (1 << kVqeLow)) zlo[tid] = uint8_t(__popc((tid ^ (tid >> 1)) & 7));
+        for (int u = tid; u < kHi; u += NT) {
+          uint32_t y, b3, x4;
+          hi_bits(u, y, b3, x4);
+          zhi[u] = uint8_t(__popc((y ^ (y >> 1)) & pmask) + int(b3 ^ x4));
+        }
+        lds_barrier();
+      }
+      // the tile in groups of 4 bits held in registers, as the rotations take it: X on
+      // a group's bits from its own 16 amplitudes (each pair once, doubled), ZZ from
+      // the first group's pass, which holds every amplitude once
       double zz_acc = 0.0, x_acc = 0.0;
+      for (int g0 = xfirst; g0 < cur.tbits; g0 += 4) {
+        const int gb = min(4, cur.tbits - g0);
+        const int nsub = TN >> gb;
+        const bool with_zz = first_e && g0 == xfirst;
+        for (int sc = tid; sc < nsub; sc += NT) {
+          const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+          V a[16];
 #pragma unroll
-      for (int k = 0; k < kPer; ++k) {
-        const int p = tid + NT * k;
-        const V v = lds[vpad<TB>(p)];
-        const double pr = double(v.x), pi = double(v.y);
-        if (first_e) zz_acc += (pr * pr + pi * pi) * zz_of(vqe_mem_index<TB>(cur, p), cur.n);
-        for (int pos = xfirst; pos < cur.tbits; ++pos) {
-          const V q = lds[vpad<TB>(p ^ (1 << pos))];
-          x_acc += pr * double(q.x) + pi * double(q.y);
+          for (int i = 0; i < 16; ++i)
+            if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+          if (with_zz) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i)
+              if (i < (1 << gb)) {
+                const int p = base + (i << g0);
+                const double pr = double(a[i].x), pi = double(a[i].y);
+                zz_acc += (pr * pr + pi * pi) *
+                          double(cur.n - 1 - 2 * (int(zlo[p & ((1 << kVqeLow) - 1)]) + int(zhi[p >> 3])));
+              }
+          }
+#pragma unroll
+          for (int k = 0; k < 4; ++k)
+            if (k < gb) {
+#pragma unroll
+              for (int i = 0; i < 16; ++i)
+                if (i < (1 << gb) && ((i >> k) & 1) == 0) {
+                  const V u = a[i], v = a[i | (1 << k)];
+                  x_acc += 2.0 * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
+                }
+            }
         }
       }
       const double zz = bred(zz_acc, red);
@@ -472,8 +540,10 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
     }
     if (j <= cur.last) {
       V* st = states + cur.state_off;
+      const uint32_t lane_bits = vqe_spread<TB>(cur, tid);
 #pragma unroll
-      for (int k = 0; k < kPer; ++k) st[vqe_mem_index<TB>(cur, tid + NT * k)] = lds[vpad<TB>(tid + NT * k)];
+      for (int k = 0; k < kPer; ++k)
+        st[cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)] = lds[vpad<TB>(tid + NT * k)];
     }
     lds_barrier();  // LDS is refilled for the next item
     if (!more) break;
