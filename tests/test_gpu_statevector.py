@@ -314,6 +314,42 @@ def test_sample_bit_exact_adversarial(gpu, case):
     assert np.array_equal(got, O.sample(n, z, None, u))
 
 
+def test_sample_marginal_and_complex64_many_buckets(gpu):
+    """The cumulative sum over more than one 4,096-bucket chunk reached through a
+    marginal distribution (13 of 15 qubits, 8,192 buckets) and through a complex64
+    register (14 qubits): every partial sum, and just below it, picks the oracle's
+    bucket."""
+    n = 15
+    psi = seeded_state(n, 81)
+    locs = [14, 0, 3, 7, 1, 12, 5, 9, 2, 11, 6, 10, 4]
+    reg = gpu.ArrayReg.from_vec(n, psi)
+    p = O.probs(n, psi, locs)
+    cs = np.cumsum(p)
+    u = np.concatenate([cs, np.nextafter(cs, 0.0)])
+    assert np.array_equal(gpu.sample(reg, locs, u), O.sample(n, psi, locs, u))
+    n = 14
+    reg = gpu.ArrayReg.from_vec(n, seeded_state(n, 82), "c64")
+    z = reg.state_vec()  # the complex64 amplitudes, exactly
+    p = O.probs(n, z)
+    cs = np.cumsum(p)
+    u = np.concatenate([cs, np.nextafter(cs, 0.0), np.random.default_rng(3).random(200)])
+    assert np.array_equal(gpu.sample(reg, None, u), O.sample(n, z, None, u))
+
+
+def test_measure_postprocess_many_buckets(gpu):
+    """measure_with_postprocess on 13 of 14 qubits (8,192 outcomes): outcome bits
+    and the collapsed state as the oracle's."""
+    n = 14
+    psi = seeded_state(n, 83)
+    locs = list(range(13))
+    for r in (0.0, 0.37, 0.999):
+        reg = gpu.ArrayReg.from_vec(n, psi)
+        bits = gpu.measure_with_postprocess(reg, locs, gpu.PostProcess.NoPostProcess, lambda: r)
+        bits_o, state_o = O.measure(n, psi, locs, 0, 0, r)
+        assert bits == bits_o
+        assert rel_err(reg.state_vec(), state_o) < 1e-12
+
+
 @pytest.mark.parametrize("post", [0, 1, 2])
 def test_measure_postprocess_vs_oracle(gpu, post):
     n = 8
