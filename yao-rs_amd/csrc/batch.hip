@@ -300,21 +300,6 @@ __device__ __forceinline__ void vqe_item(const VqeInst* insts, const VqeBig* big
     it.nh = min(cs, it.n - it.lo_bit);
   }
   it.tbits = kVqeLow + it.nh;
-  // every field is the same in all lanes: scalar registers
-  it.item = uni64(it.item);
-  it.inst = uni(it.inst);
-  it.n = uni(it.n);
-  it.depth = uni(it.depth);
-  it.step = uni(it.step);
-  it.last = uni(it.last);
-  it.chunk = uni(it.chunk);
-  it.npass = uni(it.npass);
-  it.nh = uni(it.nh);
-  it.lo_bit = uni(it.lo_bit);
-  it.tbits = uni(it.tbits);
-  it.rot_off = uni64(it.rot_off);
-  it.state_off = uni64(it.state_off);
-  it.part_off = uni64(it.part_off);
   // fixed bits: the item index above the tile's positions, with zeros inserted at 0..3
   // and at the chunk
   const uint64_t y = (uint64_t(it.item) << (TB - it.tbits)) << kVqeLow;
@@ -494,23 +479,24 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
         }
         lds_barrier();
       }
-      // the tile in groups of 4 bits held in registers, as the rotations take it: X on
-      // a group's bits from its own 16 amplitudes (each pair once, doubled), ZZ from
-      // the first group's pass, which holds every amplitude once
+      // the tile in groups of 3 bits held in registers (8 amplitudes, beside the next
+      // item's prefetched ones): X on a group's bits from its own amplitudes (each pair
+      // once, doubled), ZZ from the first group's pass, which holds every amplitude once
+      constexpr int kEb = 3;
       double zz_acc = 0.0, x_acc = 0.0;
-      for (int g0 = xfirst; g0 < cur.tbits; g0 += 4) {
-        const int gb = min(4, cur.tbits - g0);
+      for (int g0 = xfirst; g0 < cur.tbits; g0 += kEb) {
+        const int gb = min(kEb, cur.tbits - g0);
         const int nsub = TN >> gb;
         const bool with_zz = first_e && g0 == xfirst;
         for (int sc = tid; sc < nsub; sc += NT) {
           const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-          V a[16];
+          V a[1 << kEb];
 #pragma unroll
-          for (int i = 0; i < 16; ++i)
+          for (int i = 0; i < (1 << kEb); ++i)
             if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
           if (with_zz) {
 #pragma unroll
-            for (int i = 0; i < 16; ++i)
+            for (int i = 0; i < (1 << kEb); ++i)
               if (i < (1 << gb)) {
                 const int p = base + (i << g0);
                 const double pr = double(a[i].x), pi = double(a[i].y);
@@ -519,10 +505,10 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
               }
           }
 #pragma unroll
-          for (int k = 0; k < 4; ++k)
+          for (int k = 0; k < kEb; ++k)
             if (k < gb) {
 #pragma unroll
-              for (int i = 0; i < 16; ++i)
+              for (int i = 0; i < (1 << kEb); ++i)
                 if (i < (1 << gb) && ((i >> k) & 1) == 0) {
                   const V u = a[i], v = a[i | (1 << k)];
                   x_acc += 2.0 * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
