@@ -349,6 +349,9 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
   __shared__ uint8_t zlo[1 << kVqeLow];
   __shared__ uint8_t zhi[kHi];
   __shared__ V phq[64];  // e^{i theta_q} of the layer being completed, per memory bit
+  constexpr int kPh = 4;
+  __shared__ V pa[1 << kPh];
+  __shared__ V pb[1 << (TB - kVqeLow - kPh)];
   __shared__ double red[16];
   const int tid = threadIdx.x;
   int64_t w = blockIdx.x;
@@ -410,11 +413,26 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
           if (__popc(tid & (tid >> 1)) & 1) f = cmk<V>(-f.x, -f.y);
           dlo[tid] = f;
         }
+        // phases of position bits 4.. in two tables: bits 4..7 (pa) and the rest (pb, times
+        // the phases of the item's fixed bits), one product per dhi entry
+        if (tid >= 32 && tid < 32 + (1 << kPh)) {
+          const int v = tid - 32;
+          V f = cmk<V>(R(1), R(0));
+          for (uint32_t r = vqe_spread<TB>(cur, v << kVqeLow); r; r &= r - 1) f = cmul(f, phq[__builtin_ctz(r)]);
+          pa[v] = f;
+        } else if (tid >= 64 && tid < 64 + (1 << (TB - kVqeLow - kPh))) {
+          const int v = tid - 64;
+          V f = cmk<V>(R(1), R(0));
+          for (uint32_t r = cur.m0 | vqe_spread<TB>(cur, v << (kVqeLow + kPh)); r; r &= r - 1)
+            f = cmul(f, phq[__builtin_ctz(r)]);
+          pb[v] = f;
+        }
+        lds_barrier();
         for (int u = tid; u < kHi; u += NT) {
           uint32_t y, b3, x4;
           hi_bits(u, y, b3, x4);
-          V f = cmk<V>(R(1), R(0));
-          for (uint32_t r = y >> kVqeLow; r; r &= r - 1) f = cmul(f, phq[__builtin_ctz(r) + kVqeLow]);
+          const int t = u >> 1;
+          V f = cmul(pa[t & ((1 << kPh) - 1)], pb[t >> kPh]);
           if ((__popc(y & (y >> 1) & pmask) + (b3 & x4)) & 1) f = cmk<V>(-f.x, -f.y);
           dhi[u] = f;
         }
