@@ -996,7 +996,8 @@ int launch_cdf(const double* p, uint64_t n, double* cdf, void* scratch, hipStrea
     return YH_OK;
   }
   const uint64_t nch = (n + kCdfCh - 1) / kCdfCh;
-  if (nch * kCdfMaxSeg > 0x7fffffffull) return YH_ERR_INVALID_ARG;  // record slots are int
+  if (nch * kCdfMaxSeg > 0x7fffffffull)  // record slots are int
+    return fail(YH_ERR_INVALID_ARG, "distribution too large for the device cumulative sum");
   const uint64_t ns = nch * kCdfMaxSeg;
   double* tot = static_cast<double*>(scratch);
   double* base = tot + nch;
