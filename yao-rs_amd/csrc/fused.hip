@@ -203,8 +203,15 @@ void make_groups(const std::vector<TGate>& tg, const std::vector<size_t>& idx, c
       for (int k = 0; k < kGroupBits; ++k)
         if ((mg.cmask >> k) & 1) last_touch[k] = idx;
     }
-    G.count = int(micro.size()) - G.first;
-    groups.push_back(G);
+    // a group longer than the kernel's LDS window of micro gates runs as consecutive
+    // groups on the same qubits (the gates keep their order)
+    const int end = int(micro.size());
+    for (int f = G.first; f < end; f += kTileMicroWin) {
+      TileGroup part = G;
+      part.first = f;
+      part.count = std::min(kTileMicroWin, end - f);
+      groups.push_back(part);
+    }
   }
 }
 
@@ -277,6 +284,7 @@ int plan_tiles(int nbits, const yh_op* ops, size_t nops, std::vector<Step>& step
     s.pass.first_group = int(groups.size());
     make_groups(tg, idx, tpos, groups, micro);
     s.pass.ngroups = int(groups.size()) - s.pass.first_group;
+    s.pass.mg_end = int(micro.size());
     steps.push_back(s);
   }
   return YH_OK;

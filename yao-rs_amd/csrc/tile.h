@@ -32,6 +32,13 @@ struct TileGroup {
 struct TilePass {
   uint8_t hbits[kTileHigh];  // memory bit of tile qubit kTileLow + k
   int32_t first_group, ngroups;
+  int32_t mg_end;            // one past the pass's last micro gate
 };
+
+// A workgroup holds a window of this many consecutive micro gates in LDS (and the
+// pass's group descriptors, up to kTileGroupsLds); the planner splits a group with
+// more micro gates than the window holds.
+constexpr int kTileMicroWin = 30;
+constexpr int kTileGroupsLds = 64;
 
 }  // namespace yh

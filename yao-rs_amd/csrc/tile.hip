@@ -27,7 +27,7 @@ template <class R, int T>
 __device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
   using V = typename Cplx<R>::V;
   const V m0 = ldm<R>(g.m[0]), m1 = ldm<R>(g.m[1]), m2 = ldm<R>(g.m[2]), m3 = ldm<R>(g.m[3]);
-  const int cm = g.cmask, cv = g.cval;
+  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (i & (1 << T)) continue;
@@ -42,7 +42,7 @@ __device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const Mi
 
 template <class R, int T>
 __device__ __forceinline__ void mg_perm1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
-  const int cm = g.cmask, cv = g.cval;
+  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (i & (1 << T)) continue;
@@ -59,7 +59,7 @@ template <class R, int T>
 __device__ __forceinline__ void mg_diag1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
   using V = typename Cplx<R>::V;
   const V d0 = ldm<R>(g.m[0]), d1 = ldm<R>(g.m[1]);
-  const int cm = g.cmask, cv = g.cval;
+  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
 #pragma unroll
   for (int i = 0; i < 16; ++i)
     if ((i & cm) == cv) a[i] = cmul(a[i], (i & (1 << T)) ? d1 : d0);
@@ -71,7 +71,7 @@ __device__ __forceinline__ void mg_dense2(typename Cplx<R>::V (&a)[16], const Mi
   V m[16];
 #pragma unroll
   for (int e = 0; e < 16; ++e) m[e] = ldm<R>(g.m[e]);
-  const int cm = g.cmask, cv = g.cval;
+  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (i & ((1 << T0) | (1 << T1))) continue;
@@ -92,9 +92,12 @@ __device__ __forceinline__ void mg_dense2(typename Cplx<R>::V (&a)[16], const Mi
 
 template <class R>
 __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
-  switch (g.kind) {
+  // the gate is the same for every lane: its header in scalar registers, so the
+  // dispatch below is scalar branches
+  const int kind = __builtin_amdgcn_readfirstlane(g.kind);
+  switch (kind) {
     case MG_DENSE1:
-      switch (g.t0) {
+      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
         case 0: mg_dense1<R, 0>(a, g); break;
         case 1: mg_dense1<R, 1>(a, g); break;
         case 2: mg_dense1<R, 2>(a, g); break;
@@ -102,7 +105,7 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
       }
       break;
     case MG_DIAG1:
-      switch (g.t0) {
+      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
         case 0: mg_diag1<R, 0>(a, g); break;
         case 1: mg_diag1<R, 1>(a, g); break;
         case 2: mg_diag1<R, 2>(a, g); break;
@@ -110,7 +113,7 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
       }
       break;
     case MG_PERM1:
-      switch (g.t0) {
+      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
         case 0: mg_perm1<R, 0>(a, g); break;
         case 1: mg_perm1<R, 1>(a, g); break;
         case 2: mg_perm1<R, 2>(a, g); break;
@@ -118,7 +121,7 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
       }
       break;
     default:
-      switch (g.t0 * 4 + g.t1) {
+      switch (__builtin_amdgcn_readfirstlane(g.t0 * 4 + g.t1)) {
         case 1: mg_dense2<R, 0, 1>(a, g); break;
         case 2: mg_dense2<R, 0, 2>(a, g); break;
         case 3: mg_dense2<R, 0, 3>(a, g); break;
@@ -143,7 +146,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(typename Cplx<R>::V* __re
   using V = typename Cplx<R>::V;
   __shared__ V lds[kTileN + kTileN / 16];
   __shared__ uint64_t hoff[1 << kTileHigh];
+  __shared__ MicroGate win[kTileMicroWin];  // micro gates [wfirst, wfirst + kTileMicroWin)
+  __shared__ TileGroup sgr[kTileGroupsLds];
   const int tid = threadIdx.x;
+  const bool glds = pass.ngroups <= kTileGroupsLds;
+  if (glds)
+    for (int i = tid; i < pass.ngroups; i += kTileThreads) sgr[i] = groups[pass.first_group + i];
   const uint64_t base = insert_zeros(uint64_t(blockIdx.x), tile_lk.pos, tile_lk.npos);
   if (tid < (1 << kTileHigh)) {
     uint64_t o = 0;
@@ -163,11 +171,22 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(typename Cplx<R>::V* __re
   for (int k = 0; k < kAmpsPerThread; ++k) lds[lpad(tid + k * kTileThreads)] = v[k];
   __syncthreads();
 
+  int wfirst = 0, wend = 0;  // the window's micro gates (none yet)
   for (int gi = 0; gi < pass.ngroups; ++gi) {
-    const TileGroup& G = groups[pass.first_group + gi];
+    const TileGroup G = glds ? sgr[gi] : groups[pass.first_group + gi];
+    const int gfirst = __builtin_amdgcn_readfirstlane(G.first), gcount = __builtin_amdgcn_readfirstlane(G.count);
+    if (gfirst < wfirst || gfirst + gcount > wend) {  // uniform: refill the window from this group on
+      wfirst = gfirst;
+      wend = min(gfirst + kTileMicroWin, pass.mg_end);
+      const uint64_t* src = reinterpret_cast<const uint64_t*>(mg + wfirst);
+      uint64_t* dst = reinterpret_cast<uint64_t*>(win);
+      const int words = (wend - wfirst) * int(sizeof(MicroGate) / sizeof(uint64_t));
+      for (int i = tid; i < words; i += kTileThreads) dst[i] = src[i];
+      __syncthreads();
+    }
     int qs[kGroupBits];
 #pragma unroll
-    for (int k = 0; k < kGroupBits; ++k) qs[k] = G.q[k];
+    for (int k = 0; k < kGroupBits; ++k) qs[k] = __builtin_amdgcn_readfirstlane(G.q[k]);
     // sub-cube base: insert zeros at the group's tile qubits (ascending) into tid
     int sorted[kGroupBits] = {qs[0], qs[1], qs[2], qs[3]};
 #pragma unroll
@@ -189,7 +208,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile(typename Cplx<R>::V* __re
                     (((i >> 3) & 1) << qs[3]);
       a[i] = lds[lpad(p)];
     }
-    for (int k = 0; k < G.count; ++k) run_micro_gate<R>(a, mg[G.first + k]);
+    for (int k = 0; k < gcount; ++k) run_micro_gate<R>(a, win[gfirst - wfirst + k]);
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int p = b | ((i & 1) << qs[0]) | (((i >> 1) & 1) << qs[1]) | (((i >> 2) & 1) << qs[2]) |
