@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -49,6 +50,19 @@ constexpr int kVqeLow = 4;
 template <int TB>
 __device__ __forceinline__ int vpad(int p) {
   return p + (p >> 4);
+}
+
+// f(std::integral_constant<int, G>) for the G among Gs (those below TB) equal to the
+// uniform g; nothing when none is
+template <int TB, int... Gs, class F>
+__device__ __forceinline__ void with_const(int g, F&& f) {
+  (
+      [&] {
+        if constexpr (Gs < TB) {
+          if (g == Gs) f(std::integral_constant<int, Gs>{});
+        }
+      }(),
+      ...);
 }
 
 __device__ __forceinline__ double cz_sign(uint64_t x, int n) {
@@ -438,10 +452,41 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
         }
         lds_barrier();
       }
+      // one group of gb bits from position bit G0: 16 amplitudes per thread at the
+      // positions base | (i << G0), whose LDS slots are vpad(base) + vpad(i << G0) (vpad
+      // adds over disjoint bits), so one address register and immediate offsets
+      auto rot_group = [&](auto g0c, int gb, bool signs, const int (&mbk)[4]) __attribute__((always_inline)) {
+        constexpr int G0 = decltype(g0c)::value;
+        constexpr int kGb = G0 + 4 <= TB ? 4 : TB - G0;  // bits a group from G0 can hold
+        const int nsub = TN >> gb;
+        for (int sc = tid; sc < nsub; sc += NT) {
+          const int base = ((sc >> G0) << (G0 + gb)) | (sc & ((1 << G0) - 1));
+          V* const row = lds + vpad<TB>(base);
+          V a[16];
+#pragma unroll
+          for (int i = 0; i < (1 << kGb); ++i)
+            if (i < (1 << gb)) a[i] = row[vpad<TB>(i << G0)];
+#pragma unroll
+          for (int k = 0; k < kGb; ++k)
+            if (k < gb) rot_local_real<R>(a, k, ul[mbk[k] * 2]);
+          if (signs) {
+            const V* const dl = dlo + (base & ((1 << kVqeLow) - 1));
+            const V* const dh = dhi + (base >> 3);
+#pragma unroll
+            for (int i = 0; i < (1 << kGb); ++i)
+              if (i < (1 << gb))
+                row[vpad<TB>(i << G0)] =
+                    cmul(a[i], cmul(dl[(i << G0) & ((1 << kVqeLow) - 1)], dh[(i << G0) >> 3]));
+          } else {
+#pragma unroll
+            for (int i = 0; i < (1 << kGb); ++i)
+              if (i < (1 << gb)) row[vpad<TB>(i << G0)] = a[i];
+          }
+        }
+      };
       for (int g0 = first; g0 < cur.tbits; g0 += 4) {
         const int gb = min(4, cur.tbits - g0);
         const bool signs = cz && g0 + 4 >= cur.tbits;
-        const int nsub = TN >> gb;
         int mbk[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -449,24 +494,7 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
           const int mb = k < gb ? (pos < kVqeLow ? pos : cur.lo_bit + pos - kVqeLow) : 0;
           mbk[k] = __builtin_amdgcn_readfirstlane(mb);
         }
-        for (int sc = tid; sc < nsub; sc += NT) {
-          const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
-          V a[16];
-#pragma unroll
-          for (int i = 0; i < 16; ++i)
-            if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
-#pragma unroll
-          for (int k = 0; k < 4; ++k)
-            if (k < gb) rot_local_real<R>(a, k, ul[mbk[k] * 2]);
-#pragma unroll
-          for (int i = 0; i < 16; ++i)
-            if (i < (1 << gb)) {
-              const int p = base + (i << g0);
-              V v = a[i];
-              if (signs) v = cmul(v, cmul(dlo[p & ((1 << kVqeLow) - 1)], dhi[p >> 3]));
-              lds[vpad<TB>(p)] = v;
-            }
-        }
+        with_const<TB, 0, 4, 8, 12>(g0, [&](auto g0c) { rot_group(g0c, gb, signs, mbk); });
         lds_barrier();
       }
     };
@@ -502,37 +530,46 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
       // once, doubled), ZZ from the first group's pass, which holds every amplitude once
       constexpr int kEb = 3;
       double zz_acc = 0.0, x_acc = 0.0;
-      for (int g0 = xfirst; g0 < cur.tbits; g0 += kEb) {
-        const int gb = min(kEb, cur.tbits - g0);
+      // one group of gb bits from position bit G0, addressed as in the rotations
+      auto energy_group = [&](auto g0c, int gb, bool with_zz) __attribute__((always_inline)) {
+        constexpr int G0 = decltype(g0c)::value;
+        constexpr int kGb = G0 + kEb <= TB ? kEb : TB - G0;
         const int nsub = TN >> gb;
-        const bool with_zz = first_e && g0 == xfirst;
         for (int sc = tid; sc < nsub; sc += NT) {
-          const int base = ((sc >> g0) << (g0 + gb)) | (sc & ((1 << g0) - 1));
+          const int base = ((sc >> G0) << (G0 + gb)) | (sc & ((1 << G0) - 1));
+          const V* const row = lds + vpad<TB>(base);
           V a[1 << kEb];
 #pragma unroll
-          for (int i = 0; i < (1 << kEb); ++i)
-            if (i < (1 << gb)) a[i] = lds[vpad<TB>(base + (i << g0))];
+          for (int i = 0; i < (1 << kGb); ++i)
+            if (i < (1 << gb)) a[i] = row[vpad<TB>(i << G0)];
           if (with_zz) {
+            const uint8_t* const zl = zlo + (base & ((1 << kVqeLow) - 1));
+            const uint8_t* const zh = zhi + (base >> 3);
 #pragma unroll
-            for (int i = 0; i < (1 << kEb); ++i)
+            for (int i = 0; i < (1 << kGb); ++i)
               if (i < (1 << gb)) {
-                const int p = base + (i << g0);
                 const double pr = double(a[i].x), pi = double(a[i].y);
                 zz_acc += (pr * pr + pi * pi) *
-                          double(cur.n - 1 - 2 * (int(zlo[p & ((1 << kVqeLow) - 1)]) + int(zhi[p >> 3])));
+                          double(cur.n - 1 -
+                                 2 * (int(zl[(i << G0) & ((1 << kVqeLow) - 1)]) + int(zh[(i << G0) >> 3])));
               }
           }
 #pragma unroll
-          for (int k = 0; k < kEb; ++k)
+          for (int k = 0; k < kGb; ++k)
             if (k < gb) {
 #pragma unroll
-              for (int i = 0; i < (1 << kEb); ++i)
+              for (int i = 0; i < (1 << kGb); ++i)
                 if (i < (1 << gb) && ((i >> k) & 1) == 0) {
                   const V u = a[i], v = a[i | (1 << k)];
                   x_acc += 2.0 * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
                 }
             }
         }
+      };
+      for (int g0 = xfirst; g0 < cur.tbits; g0 += kEb) {
+        const int gb = min(kEb, cur.tbits - g0);
+        const bool with_zz = first_e && g0 == xfirst;
+        with_const<TB, 0, 3, 6, 9, 12, 4, 7, 10>(g0, [&](auto g0c) { energy_group(g0c, gb, with_zz); });
       }
       const double zz = bred(zz_acc, red);
       const double xx = bred(x_acc, red);
