@@ -338,8 +338,10 @@ typedef struct yh_vqe_batch yh_vqe_batch;
  * its angles_per_qubit*nq*depth angles are angles[offset[b] ...] ordered layer,
  * qubit, then the block's gates in application order.
  * Energy: TFIM H = -J sum Z_i Z_{i+1} - h sum X_i, open boundary.
- * dtype YH_C64 keeps amplitudes in complex64 and reduces in FP64; YH_C128 is the
- * all-FP64 path. The first call for a given list of (nq, depth) shapes times every
+ * dtype YH_C64 keeps amplitudes in complex64, reduces in FP64 and returns
+ * <psi|H|psi> / <psi|psi> (the FP32 roundings move the norm of the computed state,
+ * which the division takes out); YH_C128 is the all-FP64 path and returns
+ * <psi|H|psi> as expect.rs:7-24 does. The first call for a given list of (nq, depth) shapes times every
  * launch configuration on that batch and keeps the fastest for later calls with
  * the same shapes; the launch sequence is then replayed as a captured hipGraph. */
 int yh_vqe_batch_create(int max_instances, int max_qubits, int dtype, yh_vqe_batch **out);

@@ -1,9 +1,10 @@
 """Batched VQE energies (one persistent launch for many instances of uneven size)
 against the FP64 oracle and against the plain per-circuit GPU path.
-Bars: complex128 batch within 1e-10 of the oracle; complex64 batch within 1e-5 (it
-stores and rotates amplitudes in FP32: about depth x n roundings of 6e-8 per
-amplitude, so |dE| grows like 6e-8 sqrt(depth n) |E|; 3.5e-6 was the largest of 4096
-instances of 10..20 qubits, and the bench reports the fraction within 1e-6)."""
+Bars: complex128 batch within 1e-10 of the oracle; complex64 batch within 1e-6 (it
+stores and rotates amplitudes in FP32 and returns <psi|H|psi> / <psi|psi> from FP64
+sums: the FP32 roundings move the norm by up to about 1e-6, which the division
+removes; what is left is the roundings' error in the state's direction, at most
+7.6e-7 over the 4,096 instances of 10..20 qubits and depth 4..8 below)."""
 import numpy as np
 import pytest
 
@@ -40,7 +41,22 @@ def test_vqe_batch_c64_error_distribution():
     got = VqeBatch(64, 20, "c64").energies(inst)
     d = np.abs(got - want)
     print(f"c64 batch |dE|: max {d.max():.3e} median {np.median(d):.3e} p95 {np.percentile(d, 95):.3e}")
-    assert d.max() < 1e-5
+    assert d.max() < 1e-6
+
+
+def test_vqe_batch_c64_within_1e6_at_full_batch():
+    """The batched-VQE configuration at its size: 4,096 instances, n ~ U{10..20},
+    depth ~ U{4..8}, angles U[-pi, pi], TFIM J = h = 1. Every complex64 energy within
+    1e-6 of the complex128 batch's (which the tests above pin to the oracle at 1e-10),
+    LDS-resident and HBM-streamed instances alike."""
+    inst = random_instances(4096, seed=4)
+    e128 = VqeBatch(4096, 20, "c128").energies(inst)
+    e64 = VqeBatch(4096, 20, "c64").energies(inst)
+    d = np.abs(e64 - e128)
+    small = inst.nq <= 12
+    print(f"c64 vs c128, 4096 instances: max |dE| {d.max():.3e} (n <= 12: {d[small].max():.3e}, "
+          f"n > 12: {d[~small].max():.3e}), median {np.median(d):.3e}")
+    assert d.max() <= 1e-6
 
 
 def test_vqe_batch_vs_plain_path_large_instances():
@@ -99,7 +115,9 @@ def test_vqe_batch_rz_ry_rz_large_vs_plain():
     got = VqeBatch(64, 20, "c128").energies(inst)
     assert np.max(np.abs(got - plain)) < 1e-10
     got64 = VqeBatch(64, 20, "c64").energies(inst)
-    assert np.max(np.abs(got64 - plain)) < 1e-5
+    d64 = np.max(np.abs(got64 - plain))
+    print(f"Rz-Ry-Rz c64 max |dE| vs plain c128: {d64:.3e}")
+    assert d64 < 1e-6
 
 
 @pytest.mark.parametrize("apq", [2, 3])

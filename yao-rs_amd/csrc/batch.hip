@@ -47,6 +47,10 @@ namespace {
 // steps, each streaming the whole state.
 constexpr int kVqeLow = 4;
 
+// energy partial record of a (chunk, item): zz, x, and the norm (complex64 only;
+// the first energy step's items cover every amplitude once)
+constexpr int kVqePart = 3;
+
 template <int TB>
 __device__ __forceinline__ int vpad(int p) {
   return p + (p >> 4);
@@ -63,6 +67,15 @@ __device__ __forceinline__ void with_const(int g, F&& f) {
         }
       }(),
       ...);
+}
+
+// complex64 energies are <psi|H|psi> / <psi|psi>, both in FP64 from the FP32
+// amplitudes: the exact state has norm 1, and the FP32 roundings of the rotations
+// move the computed norm by up to about 1e-6, which would otherwise enter the
+// energy as a relative error of that size. complex128 energies are <psi|H|psi>.
+template <class R>
+__device__ __forceinline__ constexpr bool vqe_normalised() {
+  return sizeof(R) == 4;
 }
 
 __device__ __forceinline__ double cz_sign(uint64_t x, int n) {
@@ -202,10 +215,12 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
         lds_barrier();
       }
     }
+    double nrm_acc = 0.0;
     for (int x = tid; x < N; x += nt) {
       const V p = lds[vpad<TB>(x)];
       const double pr = double(p.x), pi = double(p.y);
       zz_acc += (pr * pr + pi * pi) * zz_of(uint64_t(x), n);
+      if (vqe_normalised<R>()) nrm_acc += pr * pr + pi * pi;
       for (int b = 0; b < n; ++b) {
         const V q = lds[vpad<TB>(x ^ (1 << b))];
         x_acc += pr * double(q.x) + pi * double(q.y);
@@ -213,7 +228,8 @@ __global__ __launch_bounds__(256, TB >= 13 ? 1 : 2) void k_vqe(const VqeInst* __
     }
     const double zz = bred(zz_acc, red);
     const double xs = bred(x_acc, red);
-    if (tid == 0) energies[in.out] = -J * zz - h * xs;
+    const double nrm = vqe_normalised<R>() ? bred(nrm_acc, red) : 1.0;
+    if (tid == 0) energies[in.out] = (-J * zz - h * xs) / nrm;
   }
 }
 
@@ -250,7 +266,7 @@ struct VqeSeg {
 };
 struct VqeBig {
   int64_t state_off;  // amplitude offset of the instance's state
-  int64_t part_off;   // offset, in (zz, x) pairs, of its energy partials
+  int64_t part_off;   // offset, in records of kVqePart doubles (zz, x, norm), of its energy partials
 };
 
 __host__ __device__ inline int vqe_npass(int n, int tb) {
@@ -529,7 +545,7 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
       // item's prefetched ones): X on a group's bits from its own amplitudes (each pair
       // once, doubled), ZZ from the first group's pass, which holds every amplitude once
       constexpr int kEb = 3;
-      double zz_acc = 0.0, x_acc = 0.0;
+      double zz_acc = 0.0, x_acc = 0.0, nrm_acc = 0.0;
       // one group of gb bits from position bit G0, addressed as in the rotations
       auto energy_group = [&](auto g0c, int gb, bool with_zz) __attribute__((always_inline)) {
         constexpr int G0 = decltype(g0c)::value;
@@ -552,6 +568,7 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
                 zz_acc += (pr * pr + pi * pi) *
                           double(cur.n - 1 -
                                  2 * (int(zl[(i << G0) & ((1 << kVqeLow) - 1)]) + int(zh[(i << G0) >> 3])));
+                if (vqe_normalised<R>()) nrm_acc += pr * pr + pi * pi;
               }
           }
 #pragma unroll
@@ -573,10 +590,13 @@ __global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ 
       }
       const double zz = bred(zz_acc, red);
       const double xx = bred(x_acc, red);
+      const double nrm = vqe_normalised<R>() ? bred(nrm_acc, red) : 0.0;
       if (tid == 0) {
-        double* o = partials + 2 * (cur.part_off + int64_t(cur.chunk) * (int64_t(1) << (cur.n - TB)) + cur.item);
+        double* o =
+            partials + kVqePart * (cur.part_off + int64_t(cur.chunk) * (int64_t(1) << (cur.n - TB)) + cur.item);
         o[0] = zz;
         o[1] = xx;
+        o[2] = nrm;
       }
     }
     if (j <= cur.last) {
@@ -706,8 +726,8 @@ __global__ __launch_bounds__(256) void k_vqe_compose(const VqeRec* __restrict__ 
 }
 
 // energies of the HBM-resident instances: one workgroup per instance folds its
-// npass * 2^(n - TB) partial pairs in a fixed order
-template <int TB>
+// npass * 2^(n - TB) partial records in a fixed order
+template <class R, int TB>
 __global__ __launch_bounds__(256) void k_vqe_energy(const VqeInst* __restrict__ insts, const VqeBig* __restrict__ big,
                                                     const double* __restrict__ partials, double* __restrict__ energies,
                                                     double J, double h) {
@@ -715,14 +735,17 @@ __global__ __launch_bounds__(256) void k_vqe_energy(const VqeInst* __restrict__ 
   const VqeInst in = insts[blockIdx.x];
   const VqeBig bg = big[blockIdx.x];
   const int64_t np = int64_t(vqe_npass(in.n, TB)) << (in.n - TB);
-  double zz = 0.0, xx = 0.0;
+  double zz = 0.0, xx = 0.0, nn = 0.0;
   for (int64_t k = threadIdx.x; k < np; k += 256) {
-    zz += partials[2 * (bg.part_off + k)];
-    xx += partials[2 * (bg.part_off + k) + 1];
+    const double* o = partials + kVqePart * (bg.part_off + k);
+    zz += o[0];
+    xx += o[1];
+    if (vqe_normalised<R>()) nn += o[2];
   }
   const double z = bred(zz, red);
   const double x = bred(xx, red);
-  if (threadIdx.x == 0) energies[in.out] = -J * z - h * x;
+  const double nrm = vqe_normalised<R>() ? bred(nn, red) : 1.0;
+  if (threadIdx.x == 0) energies[in.out] = (-J * z - h * x) / nrm;
 }
 
 __global__ void k_reset_counter(int* c) {
@@ -834,7 +857,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
   }
   const size_t amp = b->dtype == YH_C128 ? 16 : 8;
   YH_TRY(b->states.ensure(std::max<int64_t>(soff, 1) * amp));
-  YH_TRY(b->partials.ensure(std::max<int64_t>(poff, 1) * 2 * sizeof(double)));
+  YH_TRY(b->partials.ensure(std::max<int64_t>(poff, 1) * kVqePart * sizeof(double)));
   YH_TRY(b->w.upload(big.data(), big.size() * sizeof(VqeBig), b->big));
   if (!ent.empty()) YH_TRY(b->w.upload(ent.data(), ent.size() * sizeof(VqeStep), b->steps));
   if (!seg.empty()) YH_TRY(b->w.upload(seg.data(), seg.size() * sizeof(VqeSeg), b->segs));
@@ -886,7 +909,7 @@ static int vqe_launch_tb(yh_vqe_batch* b, int ninst, double J, double h, const V
   else
     YH_TRY((vqe_steps_launch<R, TB, 256>(b, cfg, ins, big, ent, sg, ryph)));
   if (b->nbig > 0)
-    hipLaunchKernelGGL(k_vqe_energy<TB>, dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
+    hipLaunchKernelGGL((k_vqe_energy<R, TB>), dim3(b->nbig), dim3(256), 0, b->w.stream, ins, big,
                        static_cast<const double*>(b->partials.p), en, J, h);
   YH_HIP(hipGetLastError());
   return YH_OK;
