@@ -355,8 +355,17 @@ __device__ __forceinline__ uint32_t vqe_spread(const VqeItem& it, int p) {
 // found by a forward scan over the launch's few segments, its entry by one shift,
 // and its amplitudes are loaded into registers while the current item is processed
 // (the barriers in the loop order LDS only, so those loads stay in flight).
+// Waves per SIMD the step kernel is compiled for: complex64 tiles of up to 12 bits
+// take at most 40.5 KB of LDS, so four 256-thread workgroups fit a CU if each
+// stays within 128 VGPRs (the compiler spills a few per-item values, stored once
+// and reloaded once per item; 11 % faster than three workgroups at 137 VGPRs).
 template <class R, int TB, int NT>
-__global__ __launch_bounds__(NT, 2) void k_vqe_step(const VqeInst* __restrict__ insts,
+constexpr int vqe_step_waves() {
+  return sizeof(R) == 4 && NT == 256 && TB <= 12 ? 4 : 2;
+}
+
+template <class R, int TB, int NT>
+__global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(const VqeInst* __restrict__ insts,
                                                      const VqeBig* __restrict__ big,
                                                      const VqeStep* __restrict__ ent,
                                                      const VqeSeg* __restrict__ seg, int nseg, int64_t nitems,
