@@ -50,6 +50,11 @@ constexpr int kVqeLow = 4;
 // energy partial record of a (chunk, item): zz, x, and the norm (complex64 only;
 // the first energy step's items cover every amplitude once)
 constexpr int kVqePart = 3;
+// records 0 .. kVqeSpare - 1 take the writes that carry nothing (see k_vqe_step),
+// record blockIdx.x % kVqeSpare for workgroup blockIdx.x, so that workgroups do not
+// all write one cache line
+constexpr int kVqeSpare = 4096;
+static_assert(kVqePart >= 2, "the spare records hold one complex128 per workgroup at 16-byte steps");
 
 template <int TB>
 __device__ __forceinline__ int vpad(int p) {
@@ -264,6 +269,8 @@ struct VqeSeg {
   int32_t ent_base;
   int32_t shift;
 };
+// segments per launch: one per qubit count above the tile size (n <= 24, TB >= 11)
+constexpr int kVqeMaxSeg = 16;
 struct VqeBig {
   int64_t state_off;  // amplitude offset of the instance's state
   int64_t part_off;   // offset, in records of kVqePart doubles (zz, x, norm), of its energy partials
@@ -395,37 +402,65 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
   const int tid = threadIdx.x;
   int64_t w = blockIdx.x;
   if (w >= nitems) return;  // uniform
+  // the launch's segments in LDS: the scan below then issues no vector loads
+  __shared__ VqeSeg sseg[kVqeMaxSeg];
+  if (tid < nseg) sseg[tid] = seg[tid];
+  __syncthreads();
   int si = 0;
   auto locate = [&](int64_t ww, VqeItem& it) {
-    while (si + 1 < nseg && seg[si + 1].item_base <= ww) ++si;
-    const VqeSeg sg = seg[si];
+    while (si + 1 < nseg && sseg[si + 1].item_base <= ww) ++si;
+    const VqeSeg sg = sseg[si];
     const int64_t rel = ww - sg.item_base;
     vqe_item<TB>(insts, big, ent[sg.ent_base + (rel >> sg.shift)], rel & ((int64_t(1) << sg.shift) - 1), it);
   };
   VqeItem cur, nxt;
   locate(w, cur);
-  // a thread's positions tid + NT k: memory index m0 | spread(tid) | spread(NT k)
+  // Every vector memory instruction of the loop below runs unconditionally and in a
+  // fixed order (the phase load, the next item's loads, the partial record, the tile
+  // stores), so the compiler's wait before the LDS fill counts only the loads it
+  // needs: a load or store under a branch makes it wait for everything issued
+  // before the branch's end, the next item's loads included, inside the item.
+  // a thread's positions tid + NT k: memory index m0 | spread(tid) | spread(NT k);
+  // a first step loads as well, every lane the instance's amplitude 0 (one cache
+  // line; the fill replaces the values)
   V pre[kPer];
-  auto load_item = [&](const VqeItem& it) __attribute__((always_inline)) {
-    const uint32_t lane_bits = vqe_spread<TB>(it, tid);
-    const V* src = states + it.state_off;
+  {
+    const uint32_t lane_bits = vqe_spread<TB>(cur, tid);
+    const uint32_t keep = cur.step == 0 ? 0u : ~0u;
+    const V* src = states + cur.state_off;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-      const uint32_t x = it.m0 | lane_bits | vqe_spread<TB>(it, NT * k);
-      pre[k] = it.step == 0 ? cmk<V>(R(x == 0 ? 1 : 0), R(0)) : src[x];
-    }
-  };
-  load_item(cur);
+    for (int k = 0; k < kPer; ++k) pre[k] = src[(cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)) & keep];
+  }
   for (;;) {
+    const int j = cur.step, k1 = cur.npass - 1;
+    // e^{i theta_q} of the layer that this step completes (thread q < n; any valid
+    // entry when the step completes none)
+    const bool czstep = j > 0 && j <= cur.last && j % k1 == 0;
+    const V phv = ryph[cur.rot_off / 2 + int64_t(czstep ? j / k1 - 1 : 0) * cur.n * 2 +
+                       2 * (cur.depth > 0 ? min(tid, cur.n - 1) : 0) + 1];
+    {
+      const uint32_t lane_bits = vqe_spread<TB>(cur, tid);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) lds[vpad<TB>(tid + NT * k)] = pre[k];
+      for (int k = 0; k < kPer; ++k) {
+        // selected per component: a select of the complex values compiles to a select
+        // of their addresses, which keeps pre[] in scratch memory
+        const R one = R((cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)) == 0);
+        const V p = pre[k];
+        lds[vpad<TB>(tid + NT * k)] = cmk<V>(j == 0 ? one : p.x, j == 0 ? R(0) : p.y);
+      }
+    }
     lds_barrier();
-    // next item: geometry, addresses and the loads, in flight during this item
+    // next item: geometry, addresses and the loads, in flight during this item (the
+    // last item of a workgroup loads itself again)
     const int64_t wn = w + gridDim.x;
     const bool more = wn < nitems;
-    if (more) {
-      locate(wn, nxt);
-      load_item(nxt);
+    locate(more ? wn : w, nxt);
+    {
+      const uint32_t lane_bits = vqe_spread<TB>(nxt, tid);
+      const uint32_t keep = nxt.step == 0 ? 0u : ~0u;
+      const V* src = states + nxt.state_off;
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) pre[k] = src[(nxt.m0 | lane_bits | vqe_spread<TB>(nxt, NT * k)) & keep];
     }
     // pairs of neighbouring memory bits from bit 4 up, of the memory index of positions
     // (u << 3) (u = p >> 3): mask of the pairs (i, i + 1) with 4 <= i <= n - 2, and the
@@ -438,11 +473,15 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
     const uint32_t pmask = ((1u << (cur.n - 1)) - 1) & ~((1u << kVqeLow) - 1);
     // layer `layer`'s Ry rotations on tile positions first..tbits-1, in groups of 4
     // bits held in registers; with `cz` the layer is complete after them, and the
-    // diagonal that ends it is applied as the last group is stored
-    auto rotate = [&](int layer, int first, bool cz) {
+    // diagonal that ends it is applied in the last group's registers. With `next`
+    // (cz too) that group then also takes layer + 1's rotations of its bits (the
+    // rotations of one layer act on distinct qubits and commute, so layer + 1 may
+    // start there), and layer + 1's other groups follow: one LDS sweep fewer.
+    auto rotate = [&](int layer, int first, bool cz, bool next) {
       const V* ul = ryph + cur.rot_off / 2 + int64_t(layer) * cur.n * 2;
+      const V* ul2 = ul + cur.n * 2;  // layer + 1
       if (cz) {
-        for (int q = tid; q < cur.n; q += NT) phq[q] = ul[2 * q + 1];
+        if (tid < cur.n) phq[tid] = phv;
         lds_barrier();
         if (tid < (1 << kVqeLow)) {
           V f = cmk<V>(R(1), R(0));
@@ -480,7 +519,8 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
       // one group of gb bits from position bit G0: 16 amplitudes per thread at the
       // positions base | (i << G0), whose LDS slots are vpad(base) + vpad(i << G0) (vpad
       // adds over disjoint bits), so one address register and immediate offsets
-      auto rot_group = [&](auto g0c, int gb, bool signs, const int (&mbk)[4]) __attribute__((always_inline)) {
+      auto rot_group = [&](auto g0c, int gb, const V* ua, bool signs, bool two, const int (&mbk)[4])
+                           __attribute__((always_inline)) {
         constexpr int G0 = decltype(g0c)::value;
         constexpr int kGb = G0 + 4 <= TB ? 4 : TB - G0;  // bits a group from G0 can hold
         const int nsub = TN >> gb;
@@ -493,25 +533,26 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
             if (i < (1 << gb)) a[i] = row[vpad<TB>(i << G0)];
 #pragma unroll
           for (int k = 0; k < kGb; ++k)
-            if (k < gb) rot_local_real<R>(a, k, ul[mbk[k] * 2]);
+            if (k < gb) rot_local_real<R>(a, k, ua[mbk[k] * 2]);
           if (signs) {
             const V* const dl = dlo + (base & ((1 << kVqeLow) - 1));
             const V* const dh = dhi + (base >> 3);
 #pragma unroll
             for (int i = 0; i < (1 << kGb); ++i)
-              if (i < (1 << gb))
-                row[vpad<TB>(i << G0)] =
-                    cmul(a[i], cmul(dl[(i << G0) & ((1 << kVqeLow) - 1)], dh[(i << G0) >> 3]));
-          } else {
+              if (i < (1 << gb)) a[i] = cmul(a[i], cmul(dl[(i << G0) & ((1 << kVqeLow) - 1)], dh[(i << G0) >> 3]));
+            if (two) {
 #pragma unroll
-            for (int i = 0; i < (1 << kGb); ++i)
-              if (i < (1 << gb)) row[vpad<TB>(i << G0)] = a[i];
+              for (int k = 0; k < kGb; ++k)
+                if (k < gb) rot_local_real<R>(a, k, ul2[mbk[k] * 2]);
+            }
           }
+#pragma unroll
+          for (int i = 0; i < (1 << kGb); ++i)
+            if (i < (1 << gb)) row[vpad<TB>(i << G0)] = a[i];
         }
       };
-      for (int g0 = first; g0 < cur.tbits; g0 += 4) {
+      auto group = [&](int g0, const V* ua, bool signs, bool two) {
         const int gb = min(4, cur.tbits - g0);
-        const bool signs = cz && g0 + 4 >= cur.tbits;
         int mbk[4];
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
@@ -519,25 +560,33 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
           const int mb = k < gb ? (pos < kVqeLow ? pos : cur.lo_bit + pos - kVqeLow) : 0;
           mbk[k] = __builtin_amdgcn_readfirstlane(mb);
         }
-        with_const<TB, 0, 4, 8, 12>(g0, [&](auto g0c) { rot_group(g0c, gb, signs, mbk); });
+        with_const<TB, 0, 4, 8, 12>(g0, [&](auto g0c) { rot_group(g0c, gb, ua, signs, two, mbk); });
         lds_barrier();
-      }
+      };
+      int glast = first;  // the layer's last group
+      while (glast + 4 < cur.tbits) glast += 4;
+      for (int g0 = first; g0 < cur.tbits; g0 += 4) group(g0, ul, cz && g0 == glast, next && g0 == glast);
+      if (next)
+        for (int g0 = 0; g0 < glast; g0 += 4) group(g0, ul2, false, false);
     };
-    const int j = cur.step, k1 = cur.npass - 1;
     if (j == 0) {
-      if (cur.depth > 0) rotate(0, 0, false);
+      if (cur.depth > 0) rotate(0, 0, false, false);
     } else if (j <= cur.last) {
       if (j % k1 == 0) {
         const int done = j / k1 - 1;
-        rotate(done, kVqeLow, true);
-        if (done + 1 < cur.depth) rotate(done + 1, 0, false);
+        rotate(done, kVqeLow, true, done + 1 < cur.depth);
       } else {
-        rotate(j / k1, kVqeLow, false);
+        rotate(j / k1, kVqeLow, false, false);
       }
     }
+    // the step's partial record: a spare record takes the (zero) record of a step
+    // without energy terms
+    const int64_t spare = blockIdx.x % kVqeSpare;
+    double pz = 0.0, px = 0.0, pn = 0.0;
+    int64_t prec = spare;
     if (j >= cur.last) {
       // energy terms of the final state: ZZ (every amplitude) and X on bits 0..3 in
-      // step `last`, X on the chunk's bits in every energy step; one partial pair
+      // step `last`, X on the chunk's bits in every energy step; one partial record
       // per (chunk, item)
       const bool first_e = j == cur.last;
       const int xfirst = first_e ? 0 : kVqeLow;
@@ -597,23 +646,28 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
         const bool with_zz = first_e && g0 == xfirst;
         with_const<TB, 0, 3, 6, 9, 12, 4, 7, 10>(g0, [&](auto g0c) { energy_group(g0c, gb, with_zz); });
       }
-      const double zz = bred(zz_acc, red);
-      const double xx = bred(x_acc, red);
-      const double nrm = vqe_normalised<R>() ? bred(nrm_acc, red) : 0.0;
-      if (tid == 0) {
-        double* o =
-            partials + kVqePart * (cur.part_off + int64_t(cur.chunk) * (int64_t(1) << (cur.n - TB)) + cur.item);
-        o[0] = zz;
-        o[1] = xx;
-        o[2] = nrm;
-      }
+      pz = bred(zz_acc, red);
+      px = bred(x_acc, red);
+      pn = vqe_normalised<R>() ? bred(nrm_acc, red) : 0.0;
+      prec = cur.part_off + int64_t(cur.chunk) * (int64_t(1) << (cur.n - TB)) + cur.item;
     }
-    if (j <= cur.last) {
-      V* st = states + cur.state_off;
+    {
+      // every thread stores the same three values (bred gives each the same sums)
+      double* o = partials + kVqePart * prec;
+      o[0] = pz;
+      o[1] = px;
+      o[2] = pn;
+    }
+    {
+      // the tile back to HBM; a step after `last` changes nothing, and its stores all
+      // go to the workgroup's spare partial record (never read)
+      const bool wr = j <= cur.last;
+      V* st = wr ? states + cur.state_off : reinterpret_cast<V*>(partials + 2 * spare);  // 16-byte aligned
+      const uint32_t keep = wr ? ~0u : 0u;
       const uint32_t lane_bits = vqe_spread<TB>(cur, tid);
 #pragma unroll
       for (int k = 0; k < kPer; ++k)
-        st[cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)] = lds[vpad<TB>(tid + NT * k)];
+        st[(cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)) & keep] = lds[vpad<TB>(tid + NT * k)];
     }
     lds_barrier();  // LDS is refilled for the next item
     if (!more) break;
@@ -835,7 +889,9 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
   int nbig = 0;
   while (nbig < ninst && b->sorted_n[nbig] > tb) ++nbig;
   std::vector<VqeBig> big(std::max(1, nbig));
-  int64_t soff = 0, poff = 0;
+  // partial records 0 .. kVqeSpare - 1: spare, written by steps without energy terms
+  // and by the tile stores of steps after the last layer
+  int64_t soff = 0, poff = kVqeSpare;
   int smax = 0;
   for (int i = 0; i < nbig; ++i) {
     const int n = b->sorted_n[i];
@@ -862,6 +918,7 @@ static int vqe_prepare(yh_vqe_batch* b, int ninst, int tb) {
         l.items += int64_t(1) << (n - tb);
       }
     }
+    if (l.nseg > kVqeMaxSeg) return fail(YH_ERR_UNSUPPORTED, "more qubit counts in one VQE step launch than its segment slots");
     L.push_back(l);
   }
   const size_t amp = b->dtype == YH_C128 ? 16 : 8;
