@@ -432,6 +432,12 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
     for (int k = 0; k < kPer; ++k) pre[k] = src[(cur.m0 | lane_bits | vqe_spread<TB>(cur, NT * k)) & keep];
   }
   for (;;) {
+    // the thread index again, through a move the compiler cannot hoist: per-thread
+    // LDS addresses derived from it are then recomputed per item rather than held
+    // for the whole loop, where they were spilled to scratch, and a scratch reload
+    // waits for every memory access before it, the next item's loads included
+    int tid;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(int(threadIdx.x)));
     const int j = cur.step, k1 = cur.npass - 1;
     // e^{i theta_q} of the layer that this step completes (thread q < n; any valid
     // entry when the step completes none)
