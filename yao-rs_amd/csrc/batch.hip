@@ -126,21 +126,22 @@ __device__ __forceinline__ void rot_group(typename Cplx<R>::V (&a)[16], int gb, 
   if (gb > 3) rot_local<R>(a, 3, u4 + 12);
 }
 
-// real rotation [[c, -s], [s, c]] on local bit k of 16 amplitudes (cs = (c, s))
+// a rotation in tangent form on local bit k of 16 amplitudes: (w, v) -> (w - tau v,
+// v + tau w), two multiply-adds per component (the record's scale and frame are
+// taken care of by the diagonal and the energy; see k_vqe_compose)
 template <class R>
-__device__ __forceinline__ void rot_local_real(typename Cplx<R>::V (&a)[16], int k, typename Cplx<R>::V cs) {
+__device__ __forceinline__ void rot_local_tan(typename Cplx<R>::V (&a)[16], int k, R tau) {
   using V = typename Cplx<R>::V;
-  const R c = cs.x, sn = cs.y;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (((i >> k) & 1) == 0) {
       const int j = i | (1 << k);
       const V w = a[i], v = a[j];
       V o0, o1;
-      o0.x = fma(c, w.x, -sn * v.x);
-      o0.y = fma(c, w.y, -sn * v.y);
-      o1.x = fma(sn, w.x, c * v.x);
-      o1.y = fma(sn, w.y, c * v.y);
+      o0.x = fma(-tau, v.x, w.x);
+      o0.y = fma(-tau, v.y, w.y);
+      o1.x = fma(tau, w.x, v.x);
+      o1.y = fma(tau, w.y, v.y);
       a[i] = o0;
       a[j] = o1;
     }
@@ -439,11 +440,22 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
     int tid;
     asm volatile("v_mov_b32 %0, %1" : "=v"(tid) : "v"(int(threadIdx.x)));
     const int j = cur.step, k1 = cur.npass - 1;
-    // e^{i theta_q} of the layer that this step completes (thread q < n; any valid
-    // entry when the step completes none)
+    // the records of the layer that this step completes (of the last layer in other
+    // steps, whose frame bits are the final frame): lane q of every wave holds qubit
+    // q's (tau, +-m) and e^{i theta_q}
     const bool czstep = j > 0 && j <= cur.last && j % k1 == 0;
-    const V phv = ryph[cur.rot_off / 2 + int64_t(czstep ? j / k1 - 1 : 0) * cur.n * 2 +
-                       2 * (cur.depth > 0 ? min(tid, cur.n - 1) : 0) + 1];
+    const int64_t rb = cur.rot_off / 2 + int64_t(czstep ? j / k1 - 1 : max(cur.depth - 1, 0)) * cur.n * 2 +
+                       2 * (cur.depth > 0 ? min(tid & 63, cur.n - 1) : 0);
+    const V rec = ryph[rb];
+    const V phv = ryph[rb + 1];
+    // frame bits by memory bit (wave-uniform) and the layer's scale S_L
+    auto frame_mask = [&]() { return uint32_t(__ballot((tid & 63) < cur.n && rec.y < R(0))); };
+    auto layer_scale = [&]() {
+      double v = (tid & 63) < cur.n ? fabs(double(rec.y)) : 1.0;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v *= __shfl_xor(v, off, 64);
+      return v;
+    };
     {
       const uint32_t lane_bits = vqe_spread<TB>(cur, tid);
 #pragma unroll
@@ -471,9 +483,10 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
     // pairs of neighbouring memory bits from bit 4 up, of the memory index of positions
     // (u << 3) (u = p >> 3): mask of the pairs (i, i + 1) with 4 <= i <= n - 2, and the
     // pair (3, 4) from position bit 3 (bit 0 of u) and memory bit 4
-    auto hi_bits = [&](int u, uint32_t& y, uint32_t& b3, uint32_t& x4) {
-      y = cur.m0 | vqe_spread<TB>(cur, (u >> 1) << kVqeLow);
-      b3 = uint32_t(u & 1);
+    // (of the index with the frame bits fm flipped)
+    auto hi_bits = [&](int u, uint32_t fm, uint32_t& y, uint32_t& b3, uint32_t& x4) {
+      y = (cur.m0 | vqe_spread<TB>(cur, (u >> 1) << kVqeLow)) ^ (fm & ~((1u << kVqeLow) - 1));
+      b3 = uint32_t(u & 1) ^ ((fm >> 3) & 1);
       x4 = (y >> kVqeLow) & 1;
     };
     const uint32_t pmask = ((1u << (cur.n - 1)) - 1) & ~((1u << kVqeLow) - 1);
@@ -487,14 +500,19 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
       const V* ul = ryph + cur.rot_off / 2 + int64_t(layer) * cur.n * 2;
       const V* ul2 = ul + cur.n * 2;  // layer + 1
       if (cz) {
-        if (tid < cur.n) phq[tid] = phv;
+        // in the frame: the diagonal at x ^ fm, i.e. the phases of frame qubits
+        // conjugated (up to a global phase) and the CZ sign of x ^ fm; times S_L
+        const uint32_t fm = frame_mask();
+        const double scl = layer_scale();
+        if (tid < cur.n) phq[tid] = ((fm >> tid) & 1) ? cmk<V>(phv.x, -phv.y) : phv;
         lds_barrier();
         if (tid < (1 << kVqeLow)) {
-          V f = cmk<V>(R(1), R(0));
+          V f = cmk<V>(R(scl), R(0));
 #pragma unroll
           for (int b = 0; b < kVqeLow; ++b)
             if ((tid >> b) & 1) f = cmul(f, phq[b]);
-          if (__popc(tid & (tid >> 1)) & 1) f = cmk<V>(-f.x, -f.y);
+          const int t = tid ^ int(fm & ((1u << kVqeLow) - 1));
+          if (__popc(t & (t >> 1)) & 1) f = cmk<V>(-f.x, -f.y);
           dlo[tid] = f;
         }
         // phases of position bits 4.. in two tables: bits 4..7 (pa) and the rest (pb, times
@@ -514,7 +532,7 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
         lds_barrier();
         for (int u = tid; u < kHi; u += NT) {
           uint32_t y, b3, x4;
-          hi_bits(u, y, b3, x4);
+          hi_bits(u, fm, y, b3, x4);
           const int t = u >> 1;
           V f = cmul(pa[t & ((1 << kPh) - 1)], pb[t >> kPh]);
           if ((__popc(y & (y >> 1) & pmask) + (b3 & x4)) & 1) f = cmk<V>(-f.x, -f.y);
@@ -539,7 +557,7 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
             if (i < (1 << gb)) a[i] = row[vpad<TB>(i << G0)];
 #pragma unroll
           for (int k = 0; k < kGb; ++k)
-            if (k < gb) rot_local_real<R>(a, k, ua[mbk[k] * 2]);
+            if (k < gb) rot_local_tan<R>(a, k, ua[mbk[k] * 2].x);
           if (signs) {
             const V* const dl = dlo + (base & ((1 << kVqeLow) - 1));
             const V* const dh = dhi + (base >> 3);
@@ -549,7 +567,7 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
             if (two) {
 #pragma unroll
               for (int k = 0; k < kGb; ++k)
-                if (k < gb) rot_local_real<R>(a, k, ul2[mbk[k] * 2]);
+                if (k < gb) rot_local_tan<R>(a, k, ul2[mbk[k] * 2].x);
             }
           }
 #pragma unroll
@@ -596,11 +614,17 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
       // per (chunk, item)
       const bool first_e = j == cur.last;
       const int xfirst = first_e ? 0 : kVqeLow;
+      // in the final frame fm: ZZ of x ^ fm, and X_q of a frame qubit with its sign
+      // changed
+      const uint32_t fm = frame_mask();
       if (first_e) {
-        if (tid < (1 << kVqeLow)) zlo[tid] = uint8_t(__popc((tid ^ (tid >> 1)) & 7));
+        if (tid < (1 << kVqeLow)) {
+          const int t = tid ^ int(fm & ((1u << kVqeLow) - 1));
+          zlo[tid] = uint8_t(__popc((t ^ (t >> 1)) & 7));
+        }
         for (int u = tid; u < kHi; u += NT) {
           uint32_t y, b3, x4;
-          hi_bits(u, y, b3, x4);
+          hi_bits(u, fm, y, b3, x4);
           zhi[u] = uint8_t(__popc((y ^ (y >> 1)) & pmask) + int(b3 ^ x4));
         }
         lds_barrier();
@@ -615,6 +639,12 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
         constexpr int G0 = decltype(g0c)::value;
         constexpr int kGb = G0 + kEb <= TB ? kEb : TB - G0;
         const int nsub = TN >> gb;
+        double sg[kEb];  // 2 X_q, with the sign of the frame, for the group's bits
+#pragma unroll
+        for (int k = 0; k < kEb; ++k) {
+          const int pos = G0 + k, mb = pos < kVqeLow ? pos : cur.lo_bit + pos - kVqeLow;
+          sg[k] = ((fm >> mb) & 1) ? -2.0 : 2.0;
+        }
         for (int sc = tid; sc < nsub; sc += NT) {
           const int base = ((sc >> G0) << (G0 + gb)) | (sc & ((1 << G0) - 1));
           const V* const row = lds + vpad<TB>(base);
@@ -642,7 +672,7 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
               for (int i = 0; i < (1 << kGb); ++i)
                 if (i < (1 << gb) && ((i >> k) & 1) == 0) {
                   const V u = a[i], v = a[i | (1 << k)];
-                  x_acc += 2.0 * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
+                  x_acc += sg[k] * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
                 }
             }
         }
@@ -748,10 +778,25 @@ __device__ void vqe_decompose(const double2 (&u)[4], double& c, double& s, doubl
 
 // Every (instance, layer, bit) record in one launch: the layer matrix for the
 // LDS-resident kernel (rot, 4 entries) and, for the step kernel (ryph, 2 entries),
-// (c, s) and e^{i theta}, theta = alpha_L + gamma_{L+1} (alpha_L for the last layer):
-// D(alpha) of layer L and D(gamma) of layer L + 1 are diagonal, commute with the CZ
-// chain between them, and merge into that one phase on bit value 1. D(gamma) of
-// layer 0 acts on |0...0> and the phases e^{i phi} are global, so both drop out.
+// the rotation's tangent form (tau, +-m) and e^{i theta}, theta = alpha_L +
+// gamma_{L+1} (alpha_L for the last layer): D(alpha) of layer L and D(gamma) of
+// layer L + 1 are diagonal, commute with the CZ chain between them, and merge into
+// that one phase on bit value 1. D(gamma) of layer 0 acts on |0...0> and the phases
+// e^{i phi} are global, so both drop out.
+//
+// Tangent form of [[c, -s], [s, c]] (c, s >= 0, c^2 + s^2 = 1) on an amplitude pair
+// (w, v): with c >= s, tau = s / c and m = c, the step kernel stores
+// (w - tau v, v + tau w) = (c w - s v, s w + c v) / m; with s > c, tau = -c / s and
+// m = s, it stores (w + (c/s) v, v - (c/s) w) = (s w + c v, -(c w - s v)) / m, the
+// rotated pair times (iY) / m. Either way two multiply-adds per component instead
+// of four operations. The factors 1 / m of a layer are global (1 / S_L, S_L the
+// product of its m), and S_L is folded into the diagonal that ends the layer. A
+// qubit's iY factors commute with every later Ry on it, and across a diagonal D
+// they turn it into D(x ^ f) (f: the qubits with an odd number of them so far), so
+// the stored state is the exact one with bits f flipped and a global phase: the
+// diagonal is taken at x ^ f (the phases of frame qubits conjugated, up to a global
+// phase), Z_q and X_q of a frame qubit change sign in the energy. The sign of m
+// carries the qubit's frame bit after the layer (negative: in the flipped frame).
 template <class R>
 __global__ __launch_bounds__(256) void k_vqe_compose(const VqeRec* __restrict__ recs, int ninst, int64_t nrec,
                                                      const double* __restrict__ angles, int apq,
@@ -780,6 +825,17 @@ __global__ __launch_bounds__(256) void k_vqe_compose(const VqeRec* __restrict__ 
     double c, s;
     double2 ea, eg;
     vqe_decompose(u, c, s, ea, eg);
+    // the qubit's frame after this layer: the parity of its layers with s > c so far
+    bool frame = s > c;
+    for (int l0 = 0; l0 < l; ++l0) {
+      double2 u0[4];
+      vqe_layer_matrix(angles + in.ang + (int64_t(l0) * in.n + loc) * apq, apq, u0);
+      double c0, s0;
+      double2 ea0, eg0;
+      vqe_decompose(u0, c0, s0, ea0, eg0);
+      frame ^= s0 > c0;
+    }
+    const double tau = s > c ? -c / s : s / c, m = s > c ? s : c;
     double2 th = ea;
     if (l + 1 < in.depth) {
       double2 un[4];
@@ -789,7 +845,7 @@ __global__ __launch_bounds__(256) void k_vqe_compose(const VqeRec* __restrict__ 
       vqe_decompose(un, c1, s1, ea1, eg1);
       th = make_double2(ea.x * eg1.x - ea.y * eg1.y, ea.x * eg1.y + ea.y * eg1.x);
     }
-    ryph[2 * r] = cmk<V>(R(c), R(s));
+    ryph[2 * r] = cmk<V>(R(tau), R(frame ? -m : m));
     ryph[2 * r + 1] = cmk<V>(R(th.x), R(th.y));
   }
 }
