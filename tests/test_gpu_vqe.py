@@ -59,6 +59,24 @@ def test_vqe_batch_c64_within_1e6_at_full_batch():
     assert d.max() <= 1e-6
 
 
+@pytest.mark.parametrize("apq", [2, 3])
+def test_vqe_batch_wide_and_deep_vs_plain(apq):
+    """22-24 qubits (three chunks at 12-bit tiles, so steps that end no layer) and
+    depth 9-12 (more layers, more rotations in the flipped frame of the tangent
+    form) against the plain per-circuit path, both dtypes, every tile size."""
+    inst = random_instances(6, seed=20 + apq, nmin=22, nmax=24, dmin=9, dmax=12, apq=apq)
+    plain = plain_energies(inst, dtype="c128")
+    for tb in (11, 12, 13):
+        got = VqeBatch(8, 24, "c128").configure(tb, 2, True, 256).energies(inst)
+        err = np.max(np.abs(got - plain))
+        print(f"apq={apq} tile_bits={tb}: c128 max |dE| vs plain {err:.3e}")
+        assert err < 1e-10
+    got64 = VqeBatch(8, 24, "c64").energies(inst)
+    d64 = np.max(np.abs(got64 - plain))
+    print(f"apq={apq}: c64 max |dE| vs plain c128 {d64:.3e}")
+    assert d64 < 1e-6
+
+
 def test_vqe_batch_vs_plain_path_large_instances():
     """Instances of 16-20 qubits (streamed through HBM tiles) against the per-circuit
     GPU path, which the state-vector tests pin to the oracle."""
