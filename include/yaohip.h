@@ -384,6 +384,9 @@ int yh_mlp_get_params(const yh_mlp *m, float *params, size_t n);
 int yh_mlp_set_params(yh_mlp *m, const float *params, size_t n);
 /* y = f(x) for n rows (any n; processed in chunks of max_batch). */
 int yh_mlp_predict(yh_mlp *m, const float *x, size_t n, float *y);
+/* The same for FP64 input rows, rounded to FP32 as they are staged (no FP32 copy
+ * of the input on the caller's side). */
+int yh_mlp_predict_f64(yh_mlp *m, const double *x, size_t n, float *y);
 /* Mean-squared-error loss of one batch (n <= max_batch) and its gradient with
  * respect to every parameter (grads may be NULL), without updating. */
 int yh_mlp_loss_grad(yh_mlp *m, const float *x, const float *y, int n, float *loss, float *grads);

@@ -104,6 +104,23 @@ def test_mlp_predict_chunked_and_params(gpu):
     assert np.array_equal(m.get_params(), p2)
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200, 1000])
+def test_mlp_predict_fp64_rows_match_fp32(gpu, n):
+    """yh_mlp_predict_f64 rounds each input to FP32 as it stages the rows: its
+    predictions equal those of the same rows rounded by numpy, bit for bit, for row
+    counts that end inside a chunk, on a chunk edge and across staging parts; and a
+    call after a larger one reuses the staging."""
+    m = Mlp(16, 64, 2, max_batch=64, seed=5)
+    x64 = np.random.default_rng(n).normal(size=(n, 16))
+    got = m.predict(x64)
+    want = m.predict(x64.astype(np.float32))
+    assert got.dtype == np.float32 and got.shape == (n,)
+    assert np.array_equal(got, want)
+    p = m.get_params().astype(np.float64)
+    ref, _, _ = O.mlp_loss_grad(16, 64, 2, p, x64.astype(np.float32), np.zeros(n, np.float32))
+    assert np.max(np.abs(got - ref)) < 1e-4 * max(1.0, np.max(np.abs(ref)))
+
+
 def test_adam_first_step_closed_form(gpu):
     # first bias-corrected Adam step: p1 = p0 - lr * g / (|g| + eps)
     m = Mlp(32, 64, 2, max_batch=128, seed=9)

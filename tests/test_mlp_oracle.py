@@ -134,3 +134,15 @@ def test_mlp_oracle_gelu_special_values():
 def test_mlp_num_params():
     assert O.mlp_num_params(288, 256, 4) == 288 * 256 + 256 + 3 * (256 * 256 + 256) + 257
     assert O.mlp_num_params(288, 256, 4, trig=True) == 576 * 256 + 256 + 3 * (256 * 256 + 256) + 257
+
+
+def test_best_k_matches_stable_argsort():
+    """The screen's selection of the k best predictions equals the first k of a
+    stable argsort, ties (many equal values, the k-th among them) included."""
+    from yaohip.surrogate import best_k
+    rng = np.random.default_rng(0)
+    for trial in range(50):
+        n = int(rng.integers(1, 400))
+        v = rng.integers(0, 6, size=n).astype(np.float64) if trial % 2 else rng.normal(size=n)
+        for k in (1, 2, n // 3 + 1, n):
+            assert np.array_equal(best_k(v, k), np.argsort(v, kind="stable")[:k])

@@ -194,6 +194,7 @@ SIGNATURES = [
     ("yh_mlp_get_params", c_int, [c_void_p, P(ctypes.c_float), c_size_t]),
     ("yh_mlp_set_params", c_int, [c_void_p, P(ctypes.c_float), c_size_t]),
     ("yh_mlp_predict", c_int, [c_void_p, P(ctypes.c_float), c_size_t, P(ctypes.c_float)]),
+    ("yh_mlp_predict_f64", c_int, [c_void_p, P(c_double), c_size_t, P(ctypes.c_float)]),
     ("yh_mlp_loss_grad", c_int, [c_void_p, P(ctypes.c_float), P(ctypes.c_float), c_int, P(ctypes.c_float),
                                  P(ctypes.c_float)]),
     ("yh_mlp_train", c_int, [c_void_p, P(ctypes.c_float), P(ctypes.c_float), c_size_t, c_int, c_int,

@@ -23,6 +23,17 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def best_k(values, k):
+    """The indices of the k smallest values, in the order of a stable sort (ties by
+    index): np.argsort(values, kind="stable")[:k] without sorting them all."""
+    values = np.asarray(values)
+    if k >= values.size:
+        return np.argsort(values, kind="stable")
+    kth = np.partition(values, k - 1)[k - 1]
+    sel = np.flatnonzero(values <= kth)  # ascending indices: ties keep index order
+    return sel[np.argsort(values[sel], kind="stable")][:k]
+
+
 class Mlp:
     """yh_mlp handle. encoding "linear" feeds the inputs as features (in_dim % 16 == 0);
     "trig" feeds (cos x_j, sin x_j) pairs (in_dim % 8 == 0). width % 64 == 0,
@@ -55,6 +66,14 @@ class Mlp:
         A.check(A.lib().yh_mlp_set_params(self._h, p.ctypes.data_as(_fp), p.size))
 
     def predict(self, x):
+        """FP64 rows go to the library as they are (it rounds them to FP32 while staging
+        them), anything else as FP32."""
+        if isinstance(x, np.ndarray) and x.dtype == np.float64:
+            x = np.ascontiguousarray(x).reshape(-1, self.in_dim)
+            y = np.zeros(x.shape[0], dtype=np.float32)
+            A.check(A.lib().yh_mlp_predict_f64(self._h, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                               x.shape[0], y.ctypes.data_as(_fp)))
+            return y
         x = _f32(x).reshape(-1, self.in_dim)
         y = np.zeros(x.shape[0], dtype=np.float32)
         A.check(A.lib().yh_mlp_predict(self._h, x.ctypes.data_as(_fp), x.shape[0], y.ctypes.data_as(_fp)))
@@ -130,7 +149,7 @@ class SurrogateScreen:
         return np.concatenate(losses)
 
     def predict(self, angles):
-        return self.mlp.predict(_f32(angles)).astype(np.float64) * self.sigma + self.mu
+        return self.mlp.predict(np.asarray(angles)).astype(np.float64) * self.sigma + self.mu
 
     def screen(self, candidates, keep_frac=0.01, dtype="c128"):
         """Ranks candidates by predicted energy, re-evaluates the best keep_frac exactly.
@@ -139,7 +158,7 @@ class SurrogateScreen:
         pred = self.predict(candidates)
         t1 = time.perf_counter()
         k = max(1, int(round(keep_frac * len(candidates))))
-        idx = np.argsort(pred, kind="stable")[:k]
+        idx = best_k(pred, k)
         exact = self.exact_energies(np.asarray(candidates)[idx], dtype)
         t2 = time.perf_counter()
         return idx, exact, pred, {"predict_s": t1 - t0, "reeval_s": t2 - t1}
