@@ -9,6 +9,9 @@
 // The arithmetic of each gate is the per-gate kernels' arithmetic (yao-rs's
 // u1rows / unrows, src/instruct_qubit.rs:23-36, 189-206); only the HBM traffic
 // is shared between gates.
+#include <type_traits>
+#include <utility>
+
 #include "kernels.h"
 #include "tile.h"
 
@@ -23,16 +26,29 @@ __device__ __forceinline__ typename Cplx<R>::V ldm(const double (&m)[2]) {
 
 __device__ __forceinline__ int lpad(int p) { return p + (p >> 4); }
 
-template <class R, int T>
+// A 1-qubit micro gate's control form C: -1 none, 0..3 one control on value 1 at
+// that local qubit, 4 any other (the gate's mask and values at run time). With C
+// known at compile time whether amplitude i is acted on is known too; the run-time
+// test of the general form costs a select per register (4 per complex128
+// amplitude) whether or not the gate has controls.
+template <int C>
+__device__ __forceinline__ bool mg_on(int i, int cm, int cv) {
+  if constexpr (C < 0) return true;
+  else if constexpr (C < 4) return (i >> C) & 1;
+  else return (i & cm) == cv;
+}
+
+template <class R, int T, int C>
 __device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
   using V = typename Cplx<R>::V;
   const V m0 = ldm<R>(g.m[0]), m1 = ldm<R>(g.m[1]), m2 = ldm<R>(g.m[2]), m3 = ldm<R>(g.m[3]);
-  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
+  const int cm = C == 4 ? __builtin_amdgcn_readfirstlane(g.cmask) : 0;
+  const int cv = C == 4 ? __builtin_amdgcn_readfirstlane(g.cval) : 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (i & (1 << T)) continue;
     const int j = i | (1 << T);
-    if ((i & cm) == cv) {
+    if (mg_on<C>(i, cm, cv)) {
       const V w = a[i], v = a[j];
       a[i] = cmul2(m0, w, m1, v);
       a[j] = cmul2(m2, w, m3, v);
@@ -40,14 +56,15 @@ __device__ __forceinline__ void mg_dense1(typename Cplx<R>::V (&a)[16], const Mi
   }
 }
 
-template <class R, int T>
+template <class R, int T, int C>
 __device__ __forceinline__ void mg_perm1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
-  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
+  const int cm = C == 4 ? __builtin_amdgcn_readfirstlane(g.cmask) : 0;
+  const int cv = C == 4 ? __builtin_amdgcn_readfirstlane(g.cval) : 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     if (i & (1 << T)) continue;
     const int j = i | (1 << T);
-    if ((i & cm) == cv) {
+    if (mg_on<C>(i, cm, cv)) {
       const typename Cplx<R>::V w = a[i];
       a[i] = a[j];
       a[j] = w;
@@ -55,14 +72,29 @@ __device__ __forceinline__ void mg_perm1(typename Cplx<R>::V (&a)[16], const Mic
   }
 }
 
-template <class R, int T>
+template <class R, int T, int C>
 __device__ __forceinline__ void mg_diag1(typename Cplx<R>::V (&a)[16], const MicroGate& g) {
   using V = typename Cplx<R>::V;
   const V d0 = ldm<R>(g.m[0]), d1 = ldm<R>(g.m[1]);
-  const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
+  const int cm = C == 4 ? __builtin_amdgcn_readfirstlane(g.cmask) : 0;
+  const int cv = C == 4 ? __builtin_amdgcn_readfirstlane(g.cval) : 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i)
-    if ((i & cm) == cv) a[i] = cmul(a[i], (i & (1 << T)) ? d1 : d0);
+    if (mg_on<C>(i, cm, cv)) a[i] = cmul(a[i], (i & (1 << T)) ? d1 : d0);
+}
+
+// f(std::integral_constant<int, V>) for the V in [0, N) equal to the uniform v
+template <int N, class F, int... Vs>
+__device__ __forceinline__ void mg_pick(int v, F&& f, std::integer_sequence<int, Vs...>) {
+  (
+      [&] {
+        if (v == Vs) f(std::integral_constant<int, Vs>{});
+      }(),
+      ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void mg_pick(int v, F&& f) {
+  mg_pick<N>(v, f, std::make_integer_sequence<int, N>{});
 }
 
 template <class R, int T0, int T1>
@@ -95,31 +127,37 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
   // the gate is the same for every lane: its header in scalar registers, so the
   // dispatch below is scalar branches
   const int kind = __builtin_amdgcn_readfirstlane(g.kind);
+  if (kind != MG_DENSE2) {
+    const int t0 = __builtin_amdgcn_readfirstlane(g.t0) & 3;
+    const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
+    // control form + 1: 0 none, 1..4 one control on value 1 (not the target), 5 other.
+    // Only uncontrolled dense gates (the merged rotations of a qubit) leave the
+    // general form: on the 28-qubit brickwork that took a pass from 3.24 to 3.15 ms,
+    // while giving the one-control forms to diagonal or permutation gates as well
+    // measured slower there (3.3-3.5 ms; the paths of the dispatch end with the 16
+    // amplitudes in different registers) for 7-10 % on a 12-qubit QFT.
+    int cf = cm == 0 ? 0 : (__popc(cm) == 1 && cv == cm && cm != (1 << t0)) ? 1 + __builtin_ctz(cm) : 5;
+    if (kind == MG_DENSE1 && cf != 0) cf = 5;
+    if (kind != MG_DENSE1) cf = 5;
+    mg_pick<4>(t0, [&](auto tc) {
+      constexpr int T = decltype(tc)::value;
+      mg_pick<6>(cf, [&](auto cc) {
+        constexpr int C = decltype(cc)::value - 1;
+        if constexpr (C >= 0 && C < 4) {
+          if constexpr (C != T) {
+            if (kind == MG_DIAG1) mg_diag1<R, T, C>(a, g);
+            else mg_perm1<R, T, C>(a, g);
+          }
+        } else {
+          if (kind == MG_DENSE1) mg_dense1<R, T, C>(a, g);
+          else if (kind == MG_DIAG1) mg_diag1<R, T, C>(a, g);
+          else mg_perm1<R, T, C>(a, g);
+        }
+      });
+    });
+    return;
+  }
   switch (kind) {
-    case MG_DENSE1:
-      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
-        case 0: mg_dense1<R, 0>(a, g); break;
-        case 1: mg_dense1<R, 1>(a, g); break;
-        case 2: mg_dense1<R, 2>(a, g); break;
-        default: mg_dense1<R, 3>(a, g); break;
-      }
-      break;
-    case MG_DIAG1:
-      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
-        case 0: mg_diag1<R, 0>(a, g); break;
-        case 1: mg_diag1<R, 1>(a, g); break;
-        case 2: mg_diag1<R, 2>(a, g); break;
-        default: mg_diag1<R, 3>(a, g); break;
-      }
-      break;
-    case MG_PERM1:
-      switch (__builtin_amdgcn_readfirstlane(g.t0)) {
-        case 0: mg_perm1<R, 0>(a, g); break;
-        case 1: mg_perm1<R, 1>(a, g); break;
-        case 2: mg_perm1<R, 2>(a, g); break;
-        default: mg_perm1<R, 3>(a, g); break;
-      }
-      break;
     default:
       switch (__builtin_amdgcn_readfirstlane(g.t0 * 4 + g.t1)) {
         case 1: mg_dense2<R, 0, 1>(a, g); break;
