@@ -365,14 +365,15 @@ __device__ __forceinline__ uint32_t vqe_spread(const VqeItem& it, int p) {
 // (the barriers in the loop order LDS only, so those loads stay in flight).
 // Waves per SIMD the step kernel is compiled for: complex64 tiles of up to 12 bits
 // take at most 40.5 KB of LDS, so four 256-thread workgroups fit a CU if each
-// stays within 128 VGPRs (the compiler spills a few per-item values, stored once
-// and reloaded once per item; 11 % faster than three workgroups at 137 VGPRs).
+// stays within 128 VGPRs (11 % faster than three workgroups).
 template <class R, int TB, int NT>
 constexpr int vqe_step_waves() {
   return sizeof(R) == 4 && NT == 256 && TB <= 12 ? 4 : 2;
 }
 
-template <class R, int TB, int NT>
+// RES: a resident grid, each workgroup walking items g, g + grid, .. with the next
+// item's amplitudes prefetched; otherwise one workgroup per item and no prefetch.
+template <class R, int TB, int NT, bool RES>
 __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(const VqeInst* __restrict__ insts,
                                                      const VqeBig* __restrict__ big,
                                                      const VqeStep* __restrict__ ent,
@@ -468,12 +469,12 @@ __global__ __launch_bounds__(NT, (vqe_step_waves<R, TB, NT>())) void k_vqe_step(
       }
     }
     lds_barrier();
-    // next item: geometry, addresses and the loads, in flight during this item (the
-    // last item of a workgroup loads itself again)
+    // next item (resident grid): geometry, addresses and the loads, in flight during
+    // this item (the last item of a workgroup loads itself again)
     const int64_t wn = w + gridDim.x;
-    const bool more = wn < nitems;
-    locate(more ? wn : w, nxt);
-    {
+    const bool more = RES && wn < nitems;
+    if constexpr (RES) {
+      locate(more ? wn : w, nxt);
       const uint32_t lane_bits = vqe_spread<TB>(nxt, tid);
       const uint32_t keep = nxt.step == 0 ? 0u : ~0u;
       const V* src = states + nxt.state_off;
@@ -1003,14 +1004,21 @@ static int vqe_steps_launch(yh_vqe_batch* b, const VqeConfig& cfg, const VqeInst
   using V = typename Cplx<R>::V;
   static int per_cu = 0;  // one per (R, TB, NT) instantiation
   if (per_cu == 0) {
-    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vqe_step<R, TB, NT>, NT, 0));
+    YH_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_vqe_step<R, TB, NT, true>, NT, 0));
     per_cu = std::max(1, per_cu);
   }
-  const int64_t resident = cfg.resident_grid ? int64_t(b->cus) * per_cu : INT64_MAX;
-  for (const auto& l : b->launches)
-    hipLaunchKernelGGL((k_vqe_step<R, TB, NT>), dim3(unsigned(std::min<int64_t>(l.items, resident))), dim3(NT), 0,
-                       b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, ryph,
-                       static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
+  if (cfg.resident_grid) {
+    const int64_t resident = int64_t(b->cus) * per_cu;
+    for (const auto& l : b->launches)
+      hipLaunchKernelGGL((k_vqe_step<R, TB, NT, true>), dim3(unsigned(std::min<int64_t>(l.items, resident))),
+                         dim3(NT), 0, b->w.stream, ins, big, ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items,
+                         ryph, static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
+  } else {
+    for (const auto& l : b->launches)
+      hipLaunchKernelGGL((k_vqe_step<R, TB, NT, false>), dim3(unsigned(l.items)), dim3(NT), 0, b->w.stream, ins, big,
+                         ent + l.entry_off, sg + l.seg_off, int(l.nseg), l.items, ryph,
+                         static_cast<V*>(b->states.p), static_cast<double*>(b->partials.p));
+  }
   return YH_OK;
 }
 
