@@ -602,9 +602,9 @@ def profile_workload(w, n, depth):
         L = A.lib()
         for loc in range(n):
             circ = Circuit.qubits(n, [put([loc], Gate.Rx(0.3))] * 4)
-            yaohip.apply_inplace(circ, reg)
+            yaohip.apply_inplace(circ, reg, 0)  # gate by gate: one k_1q launch per gate
             A.check(L.yh_reg_timing_enable(reg._h, 1))
-            yaohip.apply_inplace(circ, reg)
+            yaohip.apply_inplace(circ, reg, 0)
             c, ms = ctypes.c_uint64(), ctypes.c_double()
             A.check(L.yh_reg_timing_read(reg._h, A.K_1Q, ctypes.byref(c), ctypes.byref(ms)))
             A.check(L.yh_reg_timing_enable(reg._h, 0))

@@ -54,6 +54,49 @@ __global__ __launch_bounds__(kBlock) void k_1q(typename Cplx<R>::V* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Target bits 0..5: the partner of amplitude i, i ^ (1 << bit), sits in the same
+// aligned run of 64 amplitudes, so each lane loads one amplitude of a contiguous
+// run and takes its partner's from lane (lane ^ (1 << bit)) by a cross-lane
+// exchange inside the wave (no memory traffic). Every load and store of a wave is
+// one contiguous 1 KB run (complex128). A lane with the target bit clear computes
+// a w + b v, one with it set c w + d v (w the amplitude with the bit clear, v the
+// one with it set): the same operations in the same order as k_1q, so the results
+// are the same bit for bit. The state has at least 64 amplitudes, a multiple of 64,
+// so a wave is either wholly inside it or wholly outside.
+// ---------------------------------------------------------------------------
+constexpr int kWaveRuns = 2;  // 64-amplitude runs per wave per iteration
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_1q_wave(typename Cplx<R>::V* __restrict__ s, int bit, Mat2<R> m,
+                                                     uint64_t n) {
+  using V = typename Cplx<R>::V;
+  const int lane = threadIdx.x & 63;
+  const int x = 1 << bit;
+  const bool hi = (lane & x) != 0;
+  const V c0 = hi ? m.c : m.a, c1 = hi ? m.d : m.b;
+  const uint64_t stride = uint64_t(gridDim.x) * (kBlock * kWaveRuns);
+  for (uint64_t g0 = uint64_t(blockIdx.x) * (kBlock * kWaveRuns) + threadIdx.x; g0 < n; g0 += stride) {
+    V a[kWaveRuns];
+#pragma unroll
+    for (int u = 0; u < kWaveRuns; ++u) {
+      const uint64_t g = g0 + uint64_t(u) * kBlock;
+      if (g < n) a[u] = s[g];
+    }
+#pragma unroll
+    for (int u = 0; u < kWaveRuns; ++u) {
+      const uint64_t g = g0 + uint64_t(u) * kBlock;
+      if (g < n) {  // uniform over the wave
+        V p;
+        p.x = __shfl_xor(a[u].x, x, 64);
+        p.y = __shfl_xor(a[u].y, x, 64);
+        const V w = hi ? p : a[u], v = hi ? a[u] : p;
+        s[g] = cmul2(c0, w, c1, v);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Dense 2^NT x 2^NT on NT target bits with optional controls. Locked bits (targets
 // and controls) are inserted as zeros; control bits with value 1 are then set.
 // The matrix is row-major with gate-index bit k <-> offs[1<<k] (yao order).
@@ -149,6 +192,11 @@ static inline unsigned grid_for(uint64_t work, uint64_t per_block) { return capp
 
 template <class R>
 int launch_1q(typename Cplx<R>::V* s, int nbits, int bit, const Mat2<R>& m, hipStream_t st) {
+  if (bit < 6 && nbits >= 6) {
+    const uint64_t n = uint64_t(1) << nbits;
+    hipLaunchKernelGGL(k_1q_wave<R>, dim3(grid_for(n, kBlock * kWaveRuns)), dim3(kBlock), 0, st, s, bit, m, n);
+    return YH_OK;
+  }
   uint64_t npairs = uint64_t(1) << (nbits - 1);
   hipLaunchKernelGGL(k_1q<R>, dim3(grid_for(npairs, kBlock * kPairs)), dim3(kBlock), 0, st, s, bit, m,
                      npairs);
