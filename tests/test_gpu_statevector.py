@@ -520,3 +520,23 @@ def test_block5_matches_gate_by_gate_26q(gpu):
     a = run_gpu(gpu, n, psi, els, fused=0)
     b = run_gpu(gpu, n, psi, els, fused=5)
     assert rel_err(b, a) < TOL
+
+
+@pytest.mark.parametrize("dtype", ["c128", "c64"])
+def test_low_target_bits_in_wave_match_pair_kernel(gpu, dtype):
+    """A dense 1-qubit gate on memory bits 0..5 runs the in-wave exchange kernel,
+    on bits >= 6 the pair kernel; both do the same operations per pair, so the gate
+    on bit b equals, bit for bit, the gate on bit 8 of the state with bits b and 8
+    exchanged (exchanged back afterwards)."""
+    n = 10
+    psi = seeded_state(n, 91)
+    g = Gate.Rx(0.731)
+
+    def swap_bits(v, a, b):
+        t = np.asarray(v).reshape([2] * n)  # axis k <-> memory bit n - 1 - k
+        return np.swapaxes(t, n - 1 - a, n - 1 - b).reshape(-1)
+
+    for b in range(6):
+        got = run_gpu(gpu, n, psi, [put([n - 1 - b], g)], dtype)
+        via8 = run_gpu(gpu, n, swap_bits(psi, b, 8), [put([n - 1 - 8], g)], dtype)
+        assert np.array_equal(got, swap_bits(via8, b, 8)), f"bit {b}"
