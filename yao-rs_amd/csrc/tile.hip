@@ -26,15 +26,13 @@ __device__ __forceinline__ typename Cplx<R>::V ldm(const double (&m)[2]) {
 
 __device__ __forceinline__ int lpad(int p) { return p + (p >> 4); }
 
-// A 1-qubit micro gate's control form C: -1 none, 0..3 one control on value 1 at
-// that local qubit, 4 any other (the gate's mask and values at run time). With C
-// known at compile time whether amplitude i is acted on is known too; the run-time
-// test of the general form costs a select per register (4 per complex128
+// A 1-qubit micro gate's control form C: -1 none, otherwise the gate's control mask
+// and values tested at run time. With C = -1 every amplitude is acted on and the
+// test disappears; the run-time test costs a select per register (4 per complex128
 // amplitude) whether or not the gate has controls.
 template <int C>
 __device__ __forceinline__ bool mg_on(int i, int cm, int cv) {
   if constexpr (C < 0) return true;
-  else if constexpr (C < 4) return (i >> C) & 1;
   else return (i & cm) == cv;
 }
 
@@ -129,31 +127,20 @@ __device__ __forceinline__ void run_micro_gate(typename Cplx<R>::V (&a)[16], con
   const int kind = __builtin_amdgcn_readfirstlane(g.kind);
   if (kind != MG_DENSE2) {
     const int t0 = __builtin_amdgcn_readfirstlane(g.t0) & 3;
-    const int cm = __builtin_amdgcn_readfirstlane(g.cmask), cv = __builtin_amdgcn_readfirstlane(g.cval);
-    // control form + 1: 0 none, 1..4 one control on value 1 (not the target), 5 other.
-    // Only uncontrolled dense gates (the merged rotations of a qubit) leave the
-    // general form: on the 28-qubit brickwork that took a pass from 3.24 to 3.15 ms,
-    // while giving the one-control forms to diagonal or permutation gates as well
-    // measured slower there (3.3-3.5 ms; the paths of the dispatch end with the 16
-    // amplitudes in different registers) for 7-10 % on a 12-qubit QFT.
-    int cf = cm == 0 ? 0 : (__popc(cm) == 1 && cv == cm && cm != (1 << t0)) ? 1 + __builtin_ctz(cm) : 5;
-    if (kind == MG_DENSE1 && cf != 0) cf = 5;
-    if (kind != MG_DENSE1) cf = 5;
+    const int cm = __builtin_amdgcn_readfirstlane(g.cmask);
+    // Uncontrolled dense gates (the merged rotations of a qubit) skip the per-register
+    // control test: on the 28-qubit brickwork that took a pass from 3.24 to 3.15 ms.
+    // One-control forms for diagonal or permutation gates (control bit as a template
+    // argument) measured 7-10 % faster on a 12-qubit QFT but 5-15 % slower on the
+    // brickwork, whose CNOTs are controlled permutations, so every other gate keeps
+    // the general form.
+    const bool plain = kind == MG_DENSE1 && cm == 0;
     mg_pick<4>(t0, [&](auto tc) {
       constexpr int T = decltype(tc)::value;
-      mg_pick<6>(cf, [&](auto cc) {
-        constexpr int C = decltype(cc)::value - 1;
-        if constexpr (C >= 0 && C < 4) {
-          if constexpr (C != T) {
-            if (kind == MG_DIAG1) mg_diag1<R, T, C>(a, g);
-            else mg_perm1<R, T, C>(a, g);
-          }
-        } else {
-          if (kind == MG_DENSE1) mg_dense1<R, T, C>(a, g);
-          else if (kind == MG_DIAG1) mg_diag1<R, T, C>(a, g);
-          else mg_perm1<R, T, C>(a, g);
-        }
-      });
+      if (plain) mg_dense1<R, T, -1>(a, g);
+      else if (kind == MG_DENSE1) mg_dense1<R, T, 4>(a, g);
+      else if (kind == MG_DIAG1) mg_diag1<R, T, 4>(a, g);
+      else mg_perm1<R, T, 4>(a, g);
     });
     return;
   }
