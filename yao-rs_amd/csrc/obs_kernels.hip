@@ -507,6 +507,62 @@ __device__ __forceinline__ double walk_chain(double add, int m, double c, double
   return c;
 }
 
+// The chain of walk_chain over records of the flat list (run: the record is a binade
+// run of exponent e; lanes from m on must hold add 0). Consecutive chunks inside one
+// binade leave consecutive runs of the same exponent: each such group is summed first
+// (its shares are multiples of that binade's grid, so the sum is exact while the runs
+// fit) and costs one step of the serial chain instead of one per record. Every lane
+// then forms its record's before and after from its group's start and checks them
+// against its neighbour's (before[k] == after[k-1], before[0] == c): by induction they
+// all pass exactly when every value equals the one-by-one chain's. ok is false
+// otherwise, and the caller walks the group record by record.
+__device__ __forceinline__ double walk_merged(double add, bool run, int e, int m, double c, double& before,
+                                              double& after, bool& ok) {
+  const int lane = threadIdx.x & 63, rl = lane & 15;
+  const bool live = lane < m;
+  const int eprev = dpp_mov<0x138>(e);  // lane - 1's (wave_shr:1)
+  const int rprev = dpp_mov<0x138>(run ? 1 : 0);
+  const bool head = !live || lane == 0 || !run || rprev == 0 || eprev != e;
+  const uint64_t heads = __ballot(head);
+  const uint64_t upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
+  const int hk = 63 - __clzll((long long)(heads & upto));  // the lane's group head (lane 0 is one)
+  // sum over the group up to this lane: the steps of wave_incl_scan, each taken only
+  // from a lane of the same group
+  double incl = add, t;
+  t = dpp_mov<0x111>(incl);
+  if (rl >= 1 && lane - 1 >= hk) incl = __dadd_rn(incl, t);
+  t = dpp_mov<0x112>(incl);
+  if (rl >= 2 && lane - 2 >= hk) incl = __dadd_rn(incl, t);
+  t = dpp_mov<0x114>(incl);
+  if (rl >= 4 && lane - 4 >= hk) incl = __dadd_rn(incl, t);
+  t = dpp_mov<0x118>(incl);
+  if (rl >= 8 && lane - 8 >= hk) incl = __dadd_rn(incl, t);
+  t = dpp_mov<0x142>(incl);  // lane 15 of each row into the next row
+  if ((lane & 16) && hk < (lane & ~15)) incl = __dadd_rn(incl, t);
+  t = dpp_mov<0x143>(incl);  // lane 31 into rows 2 and 3
+  if (lane >= 32 && hk < 32) incl = __dadd_rn(incl, t);
+  const double c0 = c;
+  const uint64_t livem = m >= 64 ? ~0ull : (1ull << m) - 1;
+  uint64_t hs = heads & livem;
+  double hb = 0.0;  // c before the lane's group
+  while (hs) {
+    const int j = __builtin_ctzll(hs);
+    hs &= hs - 1;
+    const uint64_t later = j == 63 ? 0ull : heads & ~((2ull << j) - 1);
+    const int end = later ? __builtin_ctzll(later) - 1 : 63;  // the group's last lane
+    const double gsum = readlane_f64(incl, end);
+    if (lane >= j && lane <= end) hb = c;
+    c = __dadd_rn(c, gsum);
+  }
+  const double pin = dpp_mov<0x138>(incl);
+  before = lane == hk ? hb : __dadd_rn(hb, pin);
+  after = __dadd_rn(before, add);
+  const double aprev = dpp_mov<0x138>(after);
+  const bool good = !live || before == (lane == 0 ? c0 : aprev);
+  ok = __ballot(!good) == 0;
+  return readlane_f64(after, m - 1);
+}
+
 // One wave walks a chunk's nseg segments (records in L) from c: per segment, the value
 // of a single bucket or c before a run goes to sc. A group of 64 runs the addition
 // chain first and keeps it when every run fits; otherwise it is walked segment by
@@ -762,9 +818,11 @@ __global__ __launch_bounds__(kCdfT) void k_cdf_walk(const double* __restrict__ p
         }
         const int m = cnt - g < 64 ? cnt - g : 64;
         double before = 0.0, after = 0.0;
-        const double cf = walk_chain(inf < 0 ? 0.0 : add, m, c, before, after);
-        const bool bad = q < cnt && (inf < 0 || (seg_kind(inf) == kSRun && !run_fits(before, seg_exp(inf), add)));
-        if (__ballot(bad) == 0) {
+        const bool run = inf >= 0 && seg_kind(inf) == kSRun;
+        bool ok;
+        const double cf = walk_merged(inf < 0 ? 0.0 : add, run, seg_exp(inf), m, c, before, after, ok);
+        const bool bad = q < cnt && (inf < 0 || (run && !run_fits(before, seg_exp(inf), add)));
+        if (ok && __ballot(bad) == 0) {
           if (q < cnt) rc[dst] = seg_kind(inf) == kSOne ? after : before;
           c = cf;
           continue;
