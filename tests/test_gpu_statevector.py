@@ -314,6 +314,38 @@ def test_sample_bit_exact_adversarial(gpu, case):
     assert np.array_equal(got, O.sample(n, z, None, u))
 
 
+@pytest.mark.parametrize("kind", ["seeded", "uniform", "alternating chunks"])
+def test_sample_every_partial_sum_many_chunks(gpu, kind):
+    """Every partial sum of a 2^18-bucket cumulative sum (64 chunks of 4,096, whose
+    records the cross-chunk walk takes in groups of one binade) equals the sequential
+    fold's (measure.rs:140-150): uniforms at each partial sum and just below it pick
+    the bucket of the sequential loop. The uniform distribution changes binade
+    exactly at chunk edges; the third alternates chunk magnitudes by 2^20."""
+    n = 18
+    N = 1 << n
+    if kind == "seeded":
+        z = seeded_state(n, 77)
+    elif kind == "uniform":
+        z = np.full(N, 2.0 ** -9 + 0j)
+    else:
+        amp = np.where((np.arange(N) // 4096) % 2 == 0, 1.0, 2.0 ** -10)
+        z = amp * np.exp(1j * 0.37 * np.arange(N))
+        z = z / np.linalg.norm(z)
+    print(f"[case] cdf every partial sum '{kind}' n={n}", flush=True)
+    reg = gpu.ArrayReg.from_vec(n, z)
+    p = O.probs(n, z)
+    cs = np.empty(N)
+    acc = 0.0
+    for i, x in enumerate(p.tolist()):
+        acc += x
+        cs[i] = acc
+    u = np.concatenate([cs, np.nextafter(cs, 0.0)])
+    got = gpu.sample(reg, None, u)
+    want = np.searchsorted(cs, u, side="right")
+    want[want >= N] = N - 1
+    assert np.array_equal(got, want)
+
+
 def test_sample_marginal_and_complex64_many_buckets(gpu):
     """The cumulative sum over more than one 4,096-bucket chunk reached through a
     marginal distribution (13 of 15 qubits, 8,192 buckets) and through a complex64
