@@ -888,11 +888,25 @@ __global__ __launch_bounds__(kCdfT) void k_cdf_fill(const double* __restrict__ p
     L.sinfo[threadIdx.x] = rinfo[ch * kCdfMaxSeg + threadIdx.x];
   }
   __syncthreads();
+  // when the lane's buckets start 16-byte aligned (always for yh_sample: cdf sits n
+  // doubles after p, n a power of two above one chunk) and none of them is past the
+  // end or already folded, four 16-byte stores write them
+  double v[kCdfE];
+  bool keep[kCdfE], all = (reinterpret_cast<uintptr_t>(cdf + i0) & 15) == 0;
 #pragma unroll
   for (int j = 0; j < kCdfE; ++j) {
-    if (i0 + j >= n) continue;
-    if (ln.kind(j) != kBZero && seg_kind(L.sinfo[ln.seg(j)]) == kSFolded) continue;
-    cdf[i0 + j] = cdf_value(ln, j, ln.uk[j], L, sc, 0.0);
+    keep[j] = i0 + j < n && !(ln.kind(j) != kBZero && seg_kind(L.sinfo[ln.seg(j)]) == kSFolded);
+    all = all && keep[j];
+    v[j] = keep[j] ? cdf_value(ln, j, ln.uk[j], L, sc, 0.0) : 0.0;
+  }
+  if (all) {
+    double2* d = reinterpret_cast<double2*>(cdf + i0);
+#pragma unroll
+    for (int h = 0; h < kCdfE / 2; ++h) d[h] = make_double2(v[2 * h], v[2 * h + 1]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < kCdfE; ++j)
+      if (keep[j]) cdf[i0 + j] = v[j];
   }
 }
 
